@@ -1,0 +1,86 @@
+"""ctypes binding of the HIP library's C ABI (include/bcn_mi355x.h).
+
+The library is block_compression_amd/_lib/libbcn_mi355x.so, built by
+block_compression_amd/_build.py.  Loading fails loudly if it is missing:
+there is no CPU fallback behind this package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+from ._build import LIB_PATH
+
+_vp = ctypes.c_void_p
+_sz = ctypes.c_size_t
+_u32 = ctypes.c_uint32
+_i32 = ctypes.c_int
+
+BCN_OK = 0
+STATUS_NAMES = {
+    0: "BCN_OK",
+    1: "BCN_ERR_INVALID_ARGUMENT",
+    2: "BCN_ERR_NOT_MULTIPLE_OF_4",
+    3: "BCN_ERR_BUFFER_TOO_SMALL",
+    4: "BCN_ERR_TEXTURE_TOO_SMALL",
+    5: "BCN_ERR_UNSUPPORTED",
+    6: "BCN_ERR_MISALIGNED",
+    7: "BCN_ERR_HIP",
+    8: "BCN_ERR_NO_DEVICE",
+}
+
+SRC_RGBA8 = 0
+SRC_RGBA16F = 1
+
+# (name, restype, argtypes) of every exported symbol.
+SIGNATURES = [
+    ("bcn_compressor_create", _i32, [_i32, ctypes.POINTER(_vp)]),
+    ("bcn_compressor_destroy", None, [_vp]),
+    ("bcn_add_compression_task", _i32,
+     [_vp, _i32, _vp, _sz, _vp, _i32, _u32, _u32, _u32, _u32, _vp, _sz, _u32, _u32]),
+    ("bcn_compress", _i32, [_vp, _vp]),
+    ("bcn_task_count", _sz, [_vp]),
+    ("bcn_last_error", ctypes.c_char_p, [_vp]),
+    ("bcn_status_string", ctypes.c_char_p, [_i32]),
+    ("bcn_blocks_byte_size", _sz, [_i32, _u32, _u32]),
+    ("bcn_decode_rgba8", _i32, [_i32, _vp, _sz, _u32, _u32, _vp, _u32, _vp]),
+    ("bcn_decode_bc6h_rgba16f", _i32, [_vp, _sz, _u32, _u32, _i32, _vp, _u32, _vp]),
+    ("bcn_partition_tables", None, [_vp, _vp, _vp, _vp]),
+]
+
+
+class BcnError(RuntimeError):
+    """A non-zero bcn_status from the C ABI."""
+
+    def __init__(self, status: int, message: str = ""):
+        self.status = status
+        name = STATUS_NAMES.get(status, str(status))
+        super().__init__(f"{name}: {message}" if message else name)
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m block_compression_amd._build` "
+                "(or __graft_entry__.build()); there is no CPU fallback"
+            )
+        l = ctypes.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            fn = getattr(l, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+def check(status: int, handle=None) -> None:
+    if status != BCN_OK:
+        msg = ""
+        if handle is not None:
+            msg = (lib().bcn_last_error(handle) or b"").decode()
+        raise BcnError(status, msg)

@@ -1,0 +1,981 @@
+// BC7 block encoder: one 4x4 block per lane.
+//
+// Follows the reference crate's BC7 search (src/encode/bc7.rs, with the shared
+// primitives of src/encode/common.rs) decision for decision, so the emitted
+// 128-bit block is byte-identical to `compress_rgba8` on the CPU:
+//   mode 0 (+2) over partitions -> mode 1/3 (and 7) over the partitions with
+//   the smallest PCA split bound -> mode 4/5 over rotations and index
+//   precision -> mode 6, a later mode winning only with a strictly smaller
+//   error (bc7.rs:853-867).
+//
+// GPU-side choices, none of which changes a result:
+//   * texels stay packed (one RGBA8 dword per texel, 16 registers);
+//   * every error the search compares is a sum of squared differences of
+//     integers with total below 2^24 (at most 16 texels x 4 channels x 255^2),
+//     so it is exact in f32 and is computed in int32; the same holds for the
+//     masked moments and the least-squares sums (bounds stated where used);
+//   * the winner's parameters are kept and the bitstream is written once at
+//     the end; anchor index MSBs (zero after the endpoint swap) are simply not
+//     emitted, which yields the same bits as the reference's write-then-shift
+//     (common.rs:549-562);
+//   * the 64 mode-1/3 partition keys live in a caller-provided scratch row
+//     (LDS on the GPU) and the top-k are selected by ascending key; keys are
+//     unique (partition id in the low 6 bits), so this equals the reference's
+//     partial selection sort (common.rs:126-140).
+#pragma once
+#include "bcn_common.hpp"
+
+namespace bcn {
+
+// BC7Settings, 19 x u32 in the reference's repr(C) order (src/settings.rs:80-90).
+struct Bc7Params {
+  uint32_t refine_iterations[8];
+  uint32_t mode_selection[4];
+  uint32_t skip_mode2;
+  uint32_t fast_skip_threshold_mode1;
+  uint32_t fast_skip_threshold_mode3;
+  uint32_t fast_skip_threshold_mode7;
+  uint32_t mode45_channel0;
+  uint32_t refine_iterations_channel;
+  uint32_t channels;
+};
+
+// Current winner.  qep holds endpoint e (0..5) channel p (0..3) at byte e*4+p
+// of word e; for modes 4/5 the scalar channel's two endpoints sit in the
+// channel-3 bytes of endpoints 0 and 1.
+struct Bc7Best {
+  int err;
+  uint32_t mode, part, rotation, swap;
+  uint32_t qep[6];
+  uint32_t qblock[2];
+  uint32_t aqblock[2];
+};
+
+// ---- PCA fit (common.rs:294-394, 483-518) -------------------------------------
+
+// Power iteration with renormalisation every second step (common.rs:260-292).
+template <int CH>
+BCN_HD void compute_axis(float axis[4], const float c[10], int iters) {
+  float a0 = 1.0f, a1 = 1.0f, a2 = 1.0f, a3 = 1.0f;
+  for (int i = 0; i < iters; i++) {
+    float x0, x1, x2, x3 = 0.0f;
+    if (CH == 3) {
+      x0 = c[0] * a0 + c[1] * a1 + c[2] * a2;
+      x1 = c[1] * a0 + c[4] * a1 + c[5] * a2;
+      x2 = c[2] * a0 + c[5] * a1 + c[7] * a2;
+    } else {
+      x0 = c[0] * a0 + c[1] * a1 + c[2] * a2 + c[3] * a3;
+      x1 = c[1] * a0 + c[4] * a1 + c[5] * a2 + c[6] * a3;
+      x2 = c[2] * a0 + c[5] * a1 + c[7] * a2 + c[8] * a3;
+      x3 = c[3] * a0 + c[6] * a1 + c[8] * a2 + c[9] * a3;
+    }
+    a0 = x0;
+    a1 = x1;
+    a2 = x2;
+    a3 = x3;
+    if (i % 2 == 1) {
+      // The reference starts the sum at 0.0; 0.0 + x == x for x = sq(..) >= 0.
+      float norm_sq = sq(x0) + sq(x1) + sq(x2);
+      if (CH == 4) norm_sq = norm_sq + sq(x3);
+      float rnorm = 1.0f / sqrtf(norm_sq);
+      a0 *= rnorm;
+      a1 *= rnorm;
+      a2 *= rnorm;
+      a3 *= rnorm;
+    }
+  }
+  axis[0] = a0;
+  axis[1] = a1;
+  axis[2] = a2;
+  axis[3] = a3;
+}
+
+// Integer masked moments; see compute_stats_masked (common.rs:294-332).
+// Bounds: sums <= 16*255 = 4080, sums of products <= 16*255^2 = 1,040,400,
+// both < 2^24, so the reference's f32 accumulation is exact and equals these.
+struct Moments {
+  int n, s[4], q[10];
+};
+
+template <int CH>
+BCN_HD Moments moments(const uint32_t px[16], uint32_t mask) {
+  Moments m{};
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    int f = (int)((mask >> k) & 1u);
+    uint32_t t = fresh(px[k]);
+    int v0 = f ? chi(t, 0) : 0, v1 = f ? chi(t, 1) : 0, v2 = f ? chi(t, 2) : 0;
+    m.n += f;
+    m.s[0] += v0;
+    m.s[1] += v1;
+    m.s[2] += v2;
+    m.q[0] += v0 * v0;
+    m.q[1] += v0 * v1;
+    m.q[2] += v0 * v2;
+    m.q[4] += v1 * v1;
+    m.q[5] += v1 * v2;
+    m.q[7] += v2 * v2;
+    if (CH == 4) {
+      int v3 = f ? chi(t, 3) : 0;
+      m.s[3] += v3;
+      m.q[3] += v0 * v3;
+      m.q[6] += v1 * v3;
+      m.q[8] += v2 * v3;
+      m.q[9] += v3 * v3;
+    }
+  }
+  return m;
+}
+
+// covar_from_stats (common.rs:334-350), f32 with the reference's order.
+template <int CH>
+BCN_HD void covar_from_moments(float c[10], const int s[4], const int q[10], int n) {
+  float s14 = (float)n, s10 = (float)s[0], s11 = (float)s[1], s12 = (float)s[2];
+  c[0] = (float)q[0] - s10 * s10 / s14;
+  c[1] = (float)q[1] - s10 * s11 / s14;
+  c[2] = (float)q[2] - s10 * s12 / s14;
+  c[4] = (float)q[4] - s11 * s11 / s14;
+  c[5] = (float)q[5] - s11 * s12 / s14;
+  c[7] = (float)q[7] - s12 * s12 / s14;
+  if (CH == 4) {
+    float s13 = (float)s[3];
+    c[3] = (float)q[3] - s10 * s13 / s14;
+    c[6] = (float)q[6] - s11 * s13 / s14;
+    c[8] = (float)q[8] - s12 * s13 / s14;
+    c[9] = (float)q[9] - s13 * s13 / s14;
+  } else {
+    c[3] = c[6] = c[8] = c[9] = 0.0f;
+  }
+}
+
+// block_segment_core (common.rs:483-518): principal-axis endpoints of the
+// masked texels, before the BC7 clamp.
+template <int CH>
+BCN_HD void pca_segment(const uint32_t px[16], uint32_t mask, float e0[4], float e1[4]) {
+  Moments m = moments<CH>(px, mask);
+  float s14 = (float)m.n;
+  float dc[4];
+#pragma unroll
+  for (int p = 0; p < 4; p++) dc[p] = p < CH ? (float)m.s[p] / s14 : 0.0f;
+  float c[10];
+  covar_from_moments<CH>(c, m.s, m.q, m.n);
+  const float inv_var = 1.0f / (256.0f * 256.0f);
+  const float eps = 0.001f * 0.001f;
+#pragma unroll
+  for (int i = 0; i < 10; i++) c[i] *= inv_var;
+  c[0] += eps;
+  c[4] += eps;
+  c[7] += eps;
+  c[9] += eps;
+  float axis[4];
+  compute_axis<CH>(axis, c, 8);
+
+  float ext0 = INFINITY, ext1 = -INFINITY;
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    // The reference's running sum starts at 0.0; dropping that leading add
+    // can only turn a +0.0 into -0.0, and the sign of a zero projection never
+    // reaches an output (ext*axis + dc rounds it away).
+    uint32_t t = fresh(px[k]);
+    float dot = axis[0] * (chf(t, 0) - dc[0]);
+    dot = dot + axis[1] * (chf(t, 1) - dc[1]);
+    dot = dot + axis[2] * (chf(t, 2) - dc[2]);
+    if (CH == 4) dot = dot + axis[3] * (chf(t, 3) - dc[3]);
+    bool on = (mask >> k) & 1u;
+    ext0 = on ? fminf(ext0, dot) : ext0;
+    ext1 = on ? fmaxf(ext1, dot) : ext1;
+  }
+  if (ext1 - ext0 < 1.0f) {
+    ext0 -= 0.5f;
+    ext1 += 0.5f;
+  }
+#pragma unroll
+  for (int p = 0; p < 4; p++) {
+    e0[p] = p < CH ? ext0 * axis[p] + dc[p] : 0.0f;
+    e1[p] = p < CH ? ext1 * axis[p] + dc[p] : 0.0f;
+  }
+}
+
+// block_segment (bc7.rs:371-379): PCA endpoints clamped to [0, 255].
+template <int CH>
+BCN_HD void bc7_segment(const uint32_t px[16], uint32_t mask, float e0[4], float e1[4]) {
+  pca_segment<CH>(px, mask, e0, e1);
+#pragma unroll
+  for (int p = 0; p < CH; p++) {
+    e0[p] = rclamp(e0[p], 0.0f, 255.0f);
+    e1[p] = rclamp(e1[p], 0.0f, 255.0f);
+  }
+}
+
+// ---- endpoint quantisation (bc7.rs:97-237) ---------------------------------------
+
+BCN_HD int unpack_to_byte(int v, int bits) {
+  int vv = v << (8 - bits);
+  return vv + (vv >> bits);
+}
+
+// ep_quant0367 for one endpoint pair (bc7.rs:102-148) followed by ep_dequant.
+// QCH channels are quantised (4 when the alpha endpoint is emitted), ECH
+// channels enter the p-bit error.  Outputs quantised q and dequantised d.
+template <int MODE, int QCH, int ECH>
+BCN_HD void quant_pair_0367(const float e0[4], const float e1[4], int q0[4], int q1[4],
+                            int d0[4], int d1[4]) {
+  constexpr int BITS = MODE == 0 ? 4 : (MODE == 7 ? 5 : 7);
+  constexpr int L = 1 << BITS;
+  constexpr int L2 = L * 2 - 1;
+#pragma unroll
+  for (int i = 0; i < 2; i++) {
+    const float* ep = i == 0 ? e0 : e1;
+    int qb[2][4];
+    float eb[2][4];
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int p = 0; p < QCH; p++) {
+        float x = (ep[p] / 255.0f * (float)L2 - (float)b) / 2.0f + 0.5f;
+        qb[b][p] = qclamp(x, 0.0f, (float)(L - 1)) * 2 + b;
+        eb[b][p] = MODE == 0 ? (float)unpack_to_byte(qb[b][p], 5) : (float)qb[b][p];
+      }
+    float err0 = sq(ep[0] - eb[0][0]), err1 = sq(ep[0] - eb[1][0]);
+#pragma unroll
+    for (int p = 1; p < ECH; p++) {
+      err0 = err0 + sq(ep[p] - eb[0][p]);
+      err1 = err1 + sq(ep[p] - eb[1][p]);
+    }
+    bool t0 = err0 < err1;
+    int* qo = i == 0 ? q0 : q1;
+    int* dq = i == 0 ? d0 : d1;
+#pragma unroll
+    for (int p = 0; p < QCH; p++) {
+      qo[p] = t0 ? qb[0][p] : qb[1][p];
+      // ep_dequant (bc7.rs:210-232): modes 3 and 6 are 8-bit already.
+      dq[p] = MODE == 0 ? unpack_to_byte(qo[p], 5) : (MODE == 7 ? unpack_to_byte(qo[p], 6) : qo[p]);
+    }
+  }
+}
+
+// ep_quant1 (bc7.rs:150-178) for one 2-endpoint pair: one p-bit shared by the
+// pair, chosen by the RGB error of both endpoints; then ep_dequant (7 bits).
+BCN_HD void quant_pair_1(const float e0[4], const float e1[4], int q0[4], int q1[4], int d0[4],
+                         int d1[4]) {
+  int qb[2][2][3];
+  float eb[2][2][3];
+#pragma unroll
+  for (int b = 0; b < 2; b++)
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int p = 0; p < 3; p++) {
+        float v = (i == 0 ? e0[p] : e1[p]);
+        float x = (v / 255.0f * 127.0f - (float)b) / 2.0f + 0.5f;
+        qb[b][i][p] = qclamp(x, 0.0f, 63.0f) * 2 + b;
+        eb[b][i][p] = (float)unpack_to_byte(qb[b][i][p], 7);
+      }
+  float err0 = sq(e0[0] - eb[0][0][0]), err1 = sq(e0[0] - eb[1][0][0]);
+#pragma unroll
+  for (int j = 0; j < 2; j++)
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+      if (j == 0 && p == 0) continue;
+      float v = j == 0 ? e0[p] : e1[p];
+      err0 = err0 + sq(v - eb[0][j][p]);
+      err1 = err1 + sq(v - eb[1][j][p]);
+    }
+  bool t0 = err0 < err1;
+#pragma unroll
+  for (int p = 0; p < 3; p++) {
+    q0[p] = t0 ? qb[0][0][p] : qb[1][0][p];
+    q1[p] = t0 ? qb[0][1][p] : qb[1][1][p];
+    d0[p] = unpack_to_byte(q0[p], 7);
+    d1[p] = unpack_to_byte(q1[p], 7);
+  }
+}
+
+// ep_quant245 (bc7.rs:180-189) + ep_dequant: plain rounding to BITS bits.
+template <int BITS>
+BCN_HD void quant_pair_245(const float e0[4], const float e1[4], int q0[4], int q1[4], int d0[4],
+                           int d1[4]) {
+  constexpr int L = 1 << BITS;
+#pragma unroll
+  for (int p = 0; p < 3; p++) {
+    q0[p] = qclamp(e0[p] / 255.0f * (float)(L - 1) + 0.5f, 0.0f, (float)(L - 1));
+    q1[p] = qclamp(e1[p] / 255.0f * (float)(L - 1) + 0.5f, 0.0f, (float)(L - 1));
+    d0[p] = unpack_to_byte(q0[p], BITS);
+    d1[p] = unpack_to_byte(q1[p], BITS);
+  }
+}
+
+template <int MODE, int CH>
+BCN_HD void quant_pair(const float e0[4], const float e1[4], int q0[4], int q1[4], int d0[4],
+                       int d1[4]) {
+  if (MODE == 0 || MODE == 3) quant_pair_0367<MODE, 3, 3>(e0, e1, q0, q1, d0, d1);
+  else if (MODE == 7) quant_pair_0367<7, 4, 4>(e0, e1, q0, q1, d0, d1);
+  else if (MODE == 6) quant_pair_0367<6, 4, CH>(e0, e1, q0, q1, d0, d1);
+  else if (MODE == 1) quant_pair_1(e0, e1, q0, q1, d0, d1);
+  else if (MODE == 5) quant_pair_245<7>(e0, e1, q0, q1, d0, d1);
+  else quant_pair_245<5>(e0, e1, q0, q1, d0, d1);  // modes 2, 4
+}
+
+// ---- index selection (common.rs:422-481) ----------------------------------------
+
+// Index per texel against dequantised (integer) endpoints d[j][0|1][p] of
+// subset j, pattern giving each texel's subset.  Returns the exact SSE.
+// Bounds: |proj|, div <= 4*255^2 and the error sum <= 16*4*255^2 < 2^24.
+template <int CH, int BITS, int NSUB>
+BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pattern,
+                       uint32_t qblock[2]) {
+  constexpr int L = 1 << BITS;
+  int dd[3][4] = {};
+  float fdiv[3] = {1.0f, 1.0f, 1.0f};
+#pragma unroll
+  for (int j = 0; j < NSUB; j++) {
+    int div = 0;
+#pragma unroll
+    for (int p = 0; p < CH; p++) {
+      dd[j][p] = d[j][1][p] - d[j][0][p];
+      div += dd[j][p] * dd[j][p];
+    }
+    fdiv[j] = (float)div;
+  }
+  int total = 0;
+  qblock[0] = 0;
+  qblock[1] = 0;
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    int j = NSUB == 1 ? 0 : (int)((pattern >> (2 * k)) & 3u);
+    int ea[4], de[4];
+    float div;
+#pragma unroll
+    for (int p = 0; p < CH; p++) {
+      ea[p] = NSUB == 1 ? d[0][0][p] : (j == 0 ? d[0][0][p] : (j == 1 ? d[1][0][p] : d[2][0][p]));
+      de[p] = NSUB == 1 ? dd[0][p] : (j == 0 ? dd[0][p] : (j == 1 ? dd[1][p] : dd[2][p]));
+    }
+    div = NSUB == 1 ? fdiv[0] : (j == 0 ? fdiv[0] : (j == 1 ? fdiv[1] : fdiv[2]));
+    uint32_t t = fresh(px[k]);
+    int proj = 0;
+#pragma unroll
+    for (int p = 0; p < CH; p++) proj += (chi(t, p) - ea[p]) * de[p];
+    float fp = (float)proj / div;
+    int q1 = qclamp(fp * (float)L + 0.5f, 1.0f, (float)(L - 1));
+    int w0 = weight<BITS>(q1 - 1), w1 = weight<BITS>(q1);
+    int err0 = 0, err1 = 0;
+#pragma unroll
+    for (int p = 0; p < CH; p++) {
+      int eb = ea[p] + de[p];
+      int v = chi(t, p);
+      int t0 = (((64 - w0) * ea[p] + w0 * eb + 32) >> 6) - v;
+      int t1 = (((64 - w1) * ea[p] + w1 * eb + 32) >> 6) - v;
+      err0 += t0 * t0;
+      err1 += t1 * t1;
+    }
+    bool lo = err0 < err1;
+    int q = lo ? q1 - 1 : q1;
+    total += lo ? err0 : err1;
+    qblock[k >> 3] |= (uint32_t)q << (4 * (k & 7));
+  }
+  return total;
+}
+
+// opt_endpoints (common.rs:142-205): least-squares endpoints of the masked
+// texels for fixed indices.  Sums are integers (<= 16*15*255 = 61,200);
+// cxx, cyy, cxy and the determinant are exact small integers (det <=
+// 3600^2 < 2^24); the solution keeps the reference's f32 rounding order.
+template <int CH, int BITS>
+BCN_HD void opt_endpoints(const uint32_t px[16], const uint32_t qblock[2], uint32_t mask,
+                          float e0[4], float e1[4]) {
+  constexpr int L1 = (1 << BITS) - 1;
+  int sum_q = 0, sum_qq = 0, cnt = 0, sum[4] = {0, 0, 0, 0}, atb1[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    int f = (int)((mask >> k) & 1u);
+    int q = (int)((qblock[k >> 3] >> (4 * (k & 7))) & 15u);
+    int qf = f ? q : 0;
+    int xf = f ? L1 - q : 0;
+    sum_q += qf;
+    sum_qq += qf * qf;
+    cnt += f;
+    uint32_t t = fresh(px[k]);
+#pragma unroll
+    for (int p = 0; p < CH; p++) {
+      int v = f ? chi(t, p) : 0;
+      sum[p] += v;
+      atb1[p] += xf * v;
+    }
+  }
+  int cxx = cnt * L1 * L1 - 2 * L1 * sum_q + sum_qq;
+  int cyy = sum_qq;
+  int cxy = L1 * sum_q - sum_qq;
+  int det = cxx * cyy - cxy * cxy;
+  float scale = (float)L1 / (float)det;
+  float fcxx = (float)cxx, fcyy = (float)cyy, fcxy = (float)cxy;
+#pragma unroll
+  for (int p = 0; p < CH; p++) {
+    float a1 = (float)atb1[p];
+    float a2 = (float)(L1 * sum[p] - atb1[p]);
+    float x0 = (a1 * fcyy - a2 * fcxy) * scale;
+    float x1 = (a2 * fcxx - a1 * fcxy) * scale;
+    // |det| < 0.001 with an integer det: flatten to the mean.
+    float mean = (float)sum[p] / (float)cnt;
+    e0[p] = det == 0 ? mean : x0;
+    e1[p] = det == 0 ? mean : x1;
+  }
+}
+
+// ---- scalar channel of modes 4/5 (bc7.rs:239-369) ----------------------------------
+
+struct ChannelFit {
+  int err;
+  int q0, q1;
+  uint32_t qblock[2];
+};
+
+// channel_opt_quant (bc7.rs:278-320).
+template <int BITS>
+BCN_HD int channel_quant(const int cv[16], int e0, int e1, uint32_t qblock[2]) {
+  constexpr int L = 1 << BITS;
+  float den = (float)(e1 - e0) + 0.001f;
+  qblock[0] = 0;
+  qblock[1] = 0;
+  int total = 0;
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    float proj = (float)(cv[k] - e0) / den;
+    int q1 = qclamp(proj * (float)L + 0.5f, 1.0f, (float)(L - 1));
+    int w0 = weight<BITS>(q1 - 1), w1 = weight<BITS>(q1);
+    int t0 = (((64 - w0) * e0 + w0 * e1 + 32) >> 6) - cv[k];
+    int t1 = (((64 - w1) * e0 + w1 * e1 + 32) >> 6) - cv[k];
+    int err0 = t0 * t0, err1 = t1 * t1;
+    bool lo = err0 < err1;
+    total += lo ? err0 : err1;
+    qblock[k >> 3] |= (uint32_t)(lo ? q1 - 1 : q1) << (4 * (k & 7));
+  }
+  return total;
+}
+
+// channel_quant_dequant (bc7.rs:268-276) of one endpoint.
+template <int EPBITS>
+BCN_HD int channel_qd(float e, int& q) {
+  constexpr int E = 1 << EPBITS;
+  q = qclamp(e / 255.0f * (float)(E - 1) + 0.5f, 0.0f, (float)(E - 1));
+  return unpack_to_byte(q, EPBITS);
+}
+
+// opt_channel (bc7.rs:239-266) with channel_opt_endpoints (bc7.rs:322-369).
+template <int BITS, int EPBITS>
+BCN_HD ChannelFit opt_channel(const int cv[16], uint32_t refine) {
+  constexpr int L1 = (1 << BITS) - 1;
+  int mn = 255, mx = 0;
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    mn = cv[k] < mn ? cv[k] : mn;
+    mx = cv[k] > mx ? cv[k] : mx;
+  }
+  ChannelFit r;
+  int e0 = channel_qd<EPBITS>((float)mn, r.q0);
+  int e1 = channel_qd<EPBITS>((float)mx, r.q1);
+  r.err = channel_quant<BITS>(cv, e0, e1, r.qblock);
+#pragma unroll 1
+  for (uint32_t it = 0; it < refine; it++) {
+    // Sums <= 16*15*255; cxx, cyy, cxy, det exact as in opt_endpoints.
+    int sum_q = 0, sum_qq = 0, sum = 0, atb1 = 0;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      int q = (int)((r.qblock[k >> 3] >> (4 * (k & 7))) & 15u);
+      sum_q += q;
+      sum_qq += q * q;
+      sum += cv[k];
+      atb1 += (L1 - q) * cv[k];
+    }
+    int cxx = 16 * L1 * L1 - 2 * L1 * sum_q + sum_qq;
+    int cyy = sum_qq;
+    int cxy = L1 * sum_q - sum_qq;
+    int det = cxx * cyy - cxy * cxy;
+    float scale = (float)L1 / (float)det;
+    float a1 = (float)atb1, a2 = (float)(L1 * sum - atb1);
+    float x0 = (a1 * (float)cyy - a2 * (float)cxy) * scale;
+    float x1 = (a2 * (float)cxx - a1 * (float)cxy) * scale;
+    x0 = rclamp(x0, 0.0f, 255.0f);
+    x1 = rclamp(x1, 0.0f, 255.0f);
+    float mean = (float)sum / 16.0f;
+    x0 = det == 0 ? mean : x0;
+    x1 = det == 0 ? mean : x1;
+    e0 = channel_qd<EPBITS>(x0, r.q0);
+    e1 = channel_qd<EPBITS>(x1, r.q1);
+    r.err = channel_quant<BITS>(cv, e0, e1, r.qblock);
+  }
+  return r;
+}
+
+// ---- candidate evaluation ---------------------------------------------------------
+
+// A fitted candidate of modes 0-3 / 7: quantised endpoints and indices.
+struct PartFit {
+  int err;
+  uint32_t q[3][2];  // endpoint i of subset j: channel p in byte p
+  uint32_t qblock[2];
+};
+
+BCN_HD uint32_t pack4(const int q[4], int ch) {
+  uint32_t w = 0;
+#pragma unroll
+  for (int p = 0; p < 4; p++)
+    if (p < ch) w |= (uint32_t)q[p] << (8 * p);
+  return w;
+}
+
+// bc7_enc_mode01237_part_fast (bc7.rs:517-538) for partition `part_id`
+// (already offset by 64 for 3-subset modes).
+template <int MODE>
+BCN_HD void fit_partition(const uint32_t px[16], int part_id, PartFit& f) {
+  constexpr int PAIRS = (MODE == 0 || MODE == 2) ? 3 : 2;
+  constexpr int BITS = (MODE == 0 || MODE == 1) ? 3 : 2;
+  constexpr int CH = MODE == 7 ? 4 : 3;
+  int d[3][2][4];
+#pragma unroll
+  for (int j = 0; j < PAIRS; j++) {
+    float e0[4], e1[4];
+    int q0[4], q1[4];
+    bc7_segment<CH>(px, part_mask(part_id, j), e0, e1);
+    quant_pair<MODE, CH>(e0, e1, q0, q1, d[j][0], d[j][1]);
+    f.q[j][0] = pack4(q0, CH);
+    f.q[j][1] = pack4(q1, CH);
+  }
+  f.err = block_quant<CH, BITS, PAIRS>(px, d, kPart.subset[part_id], f.qblock);
+}
+
+// The refinement step of bc7_enc_mode01237 (bc7.rs:571-600).
+template <int MODE>
+BCN_HD void refine_partition(const uint32_t px[16], int part_id, const uint32_t qblock_in[2],
+                             PartFit& f) {
+  constexpr int PAIRS = (MODE == 0 || MODE == 2) ? 3 : 2;
+  constexpr int BITS = (MODE == 0 || MODE == 1) ? 3 : 2;
+  constexpr int CH = MODE == 7 ? 4 : 3;
+  int d[3][2][4];
+#pragma unroll
+  for (int j = 0; j < PAIRS; j++) {
+    float e0[4], e1[4];
+    int q0[4], q1[4];
+    opt_endpoints<CH, BITS>(px, qblock_in, part_mask(part_id, j), e0, e1);
+    quant_pair<MODE, CH>(e0, e1, q0, q1, d[j][0], d[j][1]);
+    f.q[j][0] = pack4(q0, CH);
+    f.q[j][1] = pack4(q1, CH);
+  }
+  f.err = block_quant<CH, BITS, PAIRS>(px, d, kPart.subset[part_id], f.qblock);
+}
+
+template <int MODE>
+BCN_HD void store_partfit(Bc7Best& b, const PartFit& f, int err, int part_id) {
+  constexpr int PAIRS = (MODE == 0 || MODE == 2) ? 3 : 2;
+  b.err = err;
+  b.mode = MODE;
+  b.part = (uint32_t)part_id;
+#pragma unroll
+  for (int j = 0; j < 3; j++)
+#pragma unroll
+    for (int i = 0; i < 2; i++) b.qep[j * 2 + i] = j < PAIRS ? f.q[j][i] : 0u;
+  b.qblock[0] = f.qblock[0];
+  b.qblock[1] = f.qblock[1];
+}
+
+// bc7_enc_mode01237 (bc7.rs:540-610) over `count` candidates taken from
+// `list` (a partition id in the low 6 bits of each entry, read with stride).
+template <int MODE>
+BCN_HD void enc_mode01237(const uint32_t px[16], const Bc7Params& s, const int* list, int stride,
+                          int count, bool identity_list, int opaque_err, Bc7Best& best) {
+  constexpr int PAIRS = (MODE == 0 || MODE == 2) ? 3 : 2;
+  if (count == 0) return;
+  PartFit bf;
+  int best_part = -1;
+  bf.err = 0x7FFFFFFF;
+#pragma unroll 1
+  for (int i = 0; i < count; i++) {
+    int part = identity_list ? i : (list[i * stride] & 63);
+    int part_id = PAIRS == 3 ? part + 64 : part;
+    PartFit f;
+    fit_partition<MODE>(px, part_id, f);
+    if (f.err < bf.err) {
+      bf = f;
+      best_part = part_id;
+    }
+  }
+#pragma unroll 1
+  for (uint32_t it = 0; it < s.refine_iterations[MODE]; it++) {
+    PartFit f;
+    refine_partition<MODE>(px, best_part, bf.qblock, f);
+    if (f.err < bf.err) bf = f;
+  }
+  int err = MODE != 7 ? bf.err + opaque_err : bf.err;
+  if (err < best.err) store_partfit<MODE>(best, bf, err, best_part);
+}
+
+// Write the 64 partition keys part + bound*64 (bc7.rs:629-638, 824-843).
+template <int CH>
+BCN_HD void partition_keys(const uint32_t px[16], int* keys, int stride) {
+  Moments full = moments<CH>(px, 0xFFFFu);
+#pragma unroll 1
+  for (int part = 0; part < 64; part++) {
+    uint32_t mask = kPart.mask01[part] & 0xFFFFu;
+    Moments m = moments<CH>(px, mask);
+    float bound = 0.0f;
+#pragma unroll
+    for (int side = 0; side < 2; side++) {
+      int s[4], q[10], n;
+      if (side == 0) {
+        n = m.n;
+#pragma unroll
+        for (int i = 0; i < 4; i++) s[i] = m.s[i];
+#pragma unroll
+        for (int i = 0; i < 10; i++) q[i] = m.q[i];
+      } else {
+        // full_stats - stats: exact integer differences.
+        n = full.n - m.n;
+#pragma unroll
+        for (int i = 0; i < 4; i++) s[i] = full.s[i] - m.s[i];
+#pragma unroll
+        for (int i = 0; i < 10; i++) q[i] = full.q[i] - m.q[i];
+      }
+      float c[10];
+      covar_from_moments<CH>(c, s, q, n);
+      // get_pca_bound (common.rs:208-245).
+      const float inv_var = 1.0f / (256.0f * 256.0f);
+      const float eps = 0.001f * 0.001f;
+#pragma unroll
+      for (int i = 0; i < 10; i++) c[i] *= inv_var;
+      c[0] += eps;
+      c[4] += eps;
+      c[7] += eps;
+      float axis[4];
+      compute_axis<CH>(axis, c, 4);
+      float a0, a1, a2, a3 = 0.0f;
+      if (CH == 3) {
+        a0 = c[0] * axis[0] + c[1] * axis[1] + c[2] * axis[2];
+        a1 = c[1] * axis[0] + c[4] * axis[1] + c[5] * axis[2];
+        a2 = c[2] * axis[0] + c[5] * axis[1] + c[7] * axis[2];
+      } else {
+        a0 = c[0] * axis[0] + c[1] * axis[1] + c[2] * axis[2] + c[3] * axis[3];
+        a1 = c[1] * axis[0] + c[4] * axis[1] + c[5] * axis[2] + c[6] * axis[3];
+        a2 = c[2] * axis[0] + c[5] * axis[1] + c[7] * axis[2] + c[8] * axis[3];
+        a3 = c[3] * axis[0] + c[6] * axis[1] + c[8] * axis[2] + c[9] * axis[3];
+      }
+      float sq_sum = sq(a0) + sq(a1) + sq(a2);
+      if (CH == 4) sq_sum = sq_sum + sq(a3);
+      float lambda = sqrtf(sq_sum);
+      float b = c[0] + c[4] + c[7];
+      if (CH == 4) b = b + c[9];
+      b = b - lambda;
+      b = fmaxf(b, 0.0f);
+      // `bound += b`: the first add is 0.0 + b == b for b >= +0.0.
+      bound = side == 0 ? b : bound + b;
+    }
+    float bound12 = sqrtf(bound) * 256.0f;
+    int ibound = (int)bound12;  // finite, non-negative, far below 2^31
+    keys[part * stride] = part + ibound * 64;
+  }
+}
+
+// In place: the first `count` entries become the `count` smallest keys in
+// ascending order (keys are unique).
+BCN_HD void select_smallest(int* keys, int stride, int count) {
+  int last = -2147483647 - 1;
+#pragma unroll 1
+  for (int t = 0; t < count; t++) {
+    int best = 0x7FFFFFFF, best_i = 0;
+#pragma unroll 4
+    for (int i = t; i < 64; i++) {
+      int v = keys[i * stride];
+      bool take = v < best && v > last;
+      best = take ? v : best;
+      best_i = take ? i : best_i;
+    }
+    // Swap the minimum into slot t; entries before t are all <= last.
+    int tmp = keys[t * stride];
+    keys[t * stride] = best;
+    keys[best_i * stride] = tmp;
+    last = best;
+  }
+}
+
+// bc7_enc_mode45_candidate (bc7.rs:657-733).
+template <int MODE, int CH, int SWAP>
+BCN_HD void enc_mode45_candidate(const uint32_t px[16], const Bc7Params& s, uint32_t rotation,
+                                 Bc7Best& best) {
+  constexpr int BITS = SWAP ? 3 : 2;
+  constexpr int ABITS = SWAP ? 2 : (MODE == 4 ? 3 : 2);
+  constexpr int AEPBITS = MODE == 4 ? 6 : 8;
+  uint32_t cb[16];
+  int cv[16];
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    uint32_t v = px[k] & 0x00FFFFFFu;
+    if (rotation < 3) {
+      uint32_t sh = 8u * rotation;
+      uint32_t repl = CH == 4 ? (px[k] >> 24) : 255u;
+      v = (v & ~(0xFFu << sh)) | (repl << sh);
+    }
+    cb[k] = v;
+    cv[k] = chi(px[k], (int)rotation);
+  }
+  float e0[4], e1[4];
+  bc7_segment<3>(cb, 0xFFFFu, e0, e1);
+  int q[2][4], d[3][2][4];
+  quant_pair<MODE, 3>(e0, e1, q[0], q[1], d[0][0], d[0][1]);
+  uint32_t qblock[2];
+  int err = block_quant<3, BITS, 1>(cb, d, 0u, qblock);
+#pragma unroll 1
+  for (uint32_t it = 0; it < s.refine_iterations[MODE]; it++) {
+    opt_endpoints<3, BITS>(cb, qblock, 0xFFFFu, e0, e1);
+    quant_pair<MODE, 3>(e0, e1, q[0], q[1], d[0][0], d[0][1]);
+    err = block_quant<3, BITS, 1>(cb, d, 0u, qblock);
+  }
+  ChannelFit a = opt_channel<ABITS, AEPBITS>(cv, s.refine_iterations_channel);
+  err += a.err;
+  if (err < best.err) {
+    best.err = err;
+    best.mode = MODE;
+    best.rotation = rotation;
+    best.swap = SWAP;
+    best.qep[0] = (uint32_t)q[0][0] | ((uint32_t)q[0][1] << 8) | ((uint32_t)q[0][2] << 16) |
+                  ((uint32_t)a.q0 << 24);
+    best.qep[1] = (uint32_t)q[1][0] | ((uint32_t)q[1][1] << 8) | ((uint32_t)q[1][2] << 16) |
+                  ((uint32_t)a.q1 << 24);
+    best.qblock[0] = qblock[0];
+    best.qblock[1] = qblock[1];
+    best.aqblock[0] = a.qblock[0];
+    best.aqblock[1] = a.qblock[1];
+  }
+}
+
+// bc7_enc_mode6 (bc7.rs:762-817).
+template <int CH>
+BCN_HD void enc_mode6(const uint32_t px[16], const Bc7Params& s, Bc7Best& best) {
+  float e0[4], e1[4];
+  bc7_segment<CH>(px, 0xFFFFu, e0, e1);
+  if (CH == 3) {
+    e0[3] = 255.0f;
+    e1[3] = 255.0f;
+  }
+  int q[2][4], d[3][2][4];
+  quant_pair<6, CH>(e0, e1, q[0], q[1], d[0][0], d[0][1]);
+  uint32_t qblock[2];
+  int err = block_quant<CH, 4, 1>(px, d, 0u, qblock);
+#pragma unroll 1
+  for (uint32_t it = 0; it < s.refine_iterations[6]; it++) {
+    opt_endpoints<CH, 4>(px, qblock, 0xFFFFu, e0, e1);
+    if (CH == 3) {
+      // opt_endpoints leaves the alpha endpoints alone: they keep the
+      // previous dequantised values.
+      e0[3] = (float)d[0][0][3];
+      e1[3] = (float)d[0][1][3];
+    }
+    quant_pair<6, CH>(e0, e1, q[0], q[1], d[0][0], d[0][1]);
+    err = block_quant<CH, 4, 1>(px, d, 0u, qblock);
+  }
+  if (err < best.err) {
+    best.err = err;
+    best.mode = 6;
+    best.qep[0] = (uint32_t)q[0][0] | ((uint32_t)q[0][1] << 8) | ((uint32_t)q[0][2] << 16) |
+                  ((uint32_t)q[0][3] << 24);
+    best.qep[1] = (uint32_t)q[1][0] | ((uint32_t)q[1][1] << 8) | ((uint32_t)q[1][2] << 16) |
+                  ((uint32_t)q[1][3] << 24);
+    best.qblock[0] = qblock[0];
+    best.qblock[1] = qblock[1];
+  }
+}
+
+// ---- bitstream (bc7.rs:381-515, common.rs:520-613) ------------------------------
+
+BCN_HD uint32_t qbyte(const uint32_t qep[6], int e, int p) { return (qep[e] >> (8 * p)) & 0xFFu; }
+
+// Index of texel k.
+BCN_HD uint32_t qidx(const uint32_t qb[2], int k) { return (qb[k >> 3] >> (4 * (k & 7))) & 15u; }
+
+BCN_HD void write_out(const BitWriter& w, uint32_t out[4]) {
+  out[0] = (uint32_t)w.lo;
+  out[1] = (uint32_t)(w.lo >> 32);
+  out[2] = (uint32_t)w.hi;
+  out[3] = (uint32_t)(w.hi >> 32);
+}
+
+BCN_HD void code_mode01237(const Bc7Best& b, uint32_t out[4]) {
+  const int mode = (int)b.mode;
+  const int part = (int)b.part;
+  const int bits = (mode == 0 || mode == 1) ? 3 : 2;
+  const int pairs = (mode == 0 || mode == 2) ? 3 : 2;
+  const int channels = mode == 7 ? 4 : 3;
+  const uint32_t levels = 1u << bits;
+  uint32_t qep[6];
+  for (int e = 0; e < 6; e++) qep[e] = b.qep[e];
+  // Endpoint swap so that each subset's anchor index has MSB 0
+  // (bc7_code_apply_swap_mode01237, common.rs:583-613).
+  uint32_t flips = 0;
+  for (int j = 0; j < pairs; j++) {
+    int k0 = j == 0 ? 0 : (j == 1 ? kPart.anchor1[part] : kPart.anchor2[part]);
+    if (qidx(b.qblock, k0) >= levels / 2) {
+      uint32_t t = qep[2 * j];
+      qep[2 * j] = qep[2 * j + 1];
+      qep[2 * j + 1] = t;
+      flips |= part_mask(part, j);
+    }
+  }
+  BitWriter w;
+  w.put((uint32_t)mode + 1, 1u << mode);
+  if (mode == 0) w.put(4, (uint32_t)(part & 15));
+  else w.put(6, (uint32_t)(part & 63));
+  for (int p = 0; p < channels; p++)
+    for (int e = 0; e < pairs * 2; e++) {
+      uint32_t v = qbyte(qep, e, p);
+      if (mode == 0) w.put(4, v >> 1);
+      else if (mode == 1) w.put(6, v >> 1);
+      else if (mode == 2) w.put(5, v);
+      else if (mode == 3) w.put(7, v >> 1);
+      else w.put(5, v >> 1);
+    }
+  if (mode == 1)
+    for (int j = 0; j < 2; j++) w.put(1, qbyte(qep, 2 * j, 0) & 1u);
+  if (mode == 0 || mode == 3 || mode == 7)
+    for (int e = 0; e < pairs * 2; e++) w.put(1, qbyte(qep, e, 0) & 1u);
+  int a1 = pairs > 1 ? kPart.anchor1[part] : -1;
+  int a2 = pairs > 2 ? kPart.anchor2[part] : -1;
+  for (int k = 0; k < 16; k++) {
+    uint32_t q = qidx(b.qblock, k);
+    if ((flips >> k) & 1u) q = levels - 1 - q;
+    bool anchor = k == 0 || k == a1 || k == a2;
+    w.put(anchor ? (uint32_t)bits - 1 : (uint32_t)bits, q);
+  }
+  write_out(w, out);
+}
+
+BCN_HD void code_mode45(const Bc7Best& b, uint32_t out[4]) {
+  const int mode = (int)b.mode;
+  uint32_t qep[2] = {b.qep[0] & 0xFFFFFFu, b.qep[1] & 0xFFFFFFu};
+  uint32_t aqep[2] = {b.qep[0] >> 24, b.qep[1] >> 24};
+  uint32_t qblock[2] = {b.qblock[0], b.qblock[1]};
+  uint32_t aqblock[2] = {b.aqblock[0], b.aqblock[1]};
+  const uint32_t bits = 2, abits = mode == 4 ? 3 : 2;
+  const uint32_t epbits = mode == 4 ? 5 : 7, aepbits = mode == 4 ? 6 : 8;
+  // bc7_code_apply_swap_mode456 (common.rs:564-581); with swap set the vector
+  // and scalar index sets trade places first (bc7.rs:455-463).
+  if (b.swap) {
+    uint32_t t0 = qblock[0], t1 = qblock[1];
+    qblock[0] = aqblock[0];
+    qblock[1] = aqblock[1];
+    aqblock[0] = t0;
+    aqblock[1] = t1;
+  }
+  uint32_t* vq = b.swap ? aqep : qep;        // endpoints tied to qblock's swap test
+  uint32_t* sq2 = b.swap ? qep : aqep;       // endpoints tied to aqblock's swap test
+  uint32_t lv = 1u << bits, la = 1u << abits;
+  if ((qblock[0] & 15u) >= lv / 2) {
+    uint32_t t = vq[0];
+    vq[0] = vq[1];
+    vq[1] = t;
+    for (int i = 0; i < 2; i++) qblock[i] = 0x11111111u * (lv - 1) - qblock[i];
+  }
+  if ((aqblock[0] & 15u) >= la / 2) {
+    uint32_t t = sq2[0];
+    sq2[0] = sq2[1];
+    sq2[1] = t;
+    for (int i = 0; i < 2; i++) aqblock[i] = 0x11111111u * (la - 1) - aqblock[i];
+  }
+  BitWriter w;
+  w.put((uint32_t)mode + 1, 1u << mode);
+  w.put(2, (b.rotation + 1) & 3u);
+  if (mode == 4) w.put(1, b.swap);
+  for (int p = 0; p < 3; p++) {
+    w.put(epbits, (qep[0] >> (8 * p)) & 0xFFu);
+    w.put(epbits, (qep[1] >> (8 * p)) & 0xFFu);
+  }
+  w.put(aepbits, aqep[0]);
+  w.put(aepbits, aqep[1]);
+  for (int k = 0; k < 16; k++) w.put(k == 0 ? bits - 1 : bits, qidx(qblock, k));
+  for (int k = 0; k < 16; k++) w.put(k == 0 ? abits - 1 : abits, qidx(aqblock, k));
+  write_out(w, out);
+}
+
+BCN_HD void code_mode6(const Bc7Best& b, uint32_t out[4]) {
+  uint32_t qep[2] = {b.qep[0], b.qep[1]};
+  uint32_t qblock[2] = {b.qblock[0], b.qblock[1]};
+  if ((qblock[0] & 15u) >= 8u) {
+    uint32_t t = qep[0];
+    qep[0] = qep[1];
+    qep[1] = t;
+    for (int i = 0; i < 2; i++) qblock[i] = 0x11111111u * 15u - qblock[i];
+  }
+  BitWriter w;
+  w.put(7, 64);
+  for (int p = 0; p < 4; p++) {
+    w.put(7, ((qep[0] >> (8 * p)) & 0xFFu) >> 1);
+    w.put(7, ((qep[1] >> (8 * p)) & 0xFFu) >> 1);
+  }
+  w.put(1, qep[0] & 1u);
+  w.put(1, qep[1] & 1u);
+  for (int k = 0; k < 16; k++) w.put(k == 0 ? 3 : 4, qidx(qblock, k));
+  write_out(w, out);
+}
+
+// compress_block_bc7_core (bc7.rs:853-867) for one block.  `keys` is a
+// 64-entry scratch row read and written with `stride`.
+template <int CH>
+BCN_HD void encode_bc7_block(const uint32_t px[16], const Bc7Params& s, int* keys, int stride,
+                             uint32_t out[4]) {
+  Bc7Best best;
+  best.err = 0x7FFFFFFF;  // stands for the reference's f32::INFINITY
+  best.mode = 8;
+  best.part = best.rotation = best.swap = 0;
+  for (int e = 0; e < 6; e++) best.qep[e] = 0;
+  best.qblock[0] = best.qblock[1] = best.aqblock[0] = best.aqblock[1] = 0;
+
+  // compute_opaque_err (bc7.rs:869-879): exact integer (<= 16*255^2).
+  int opaque_err = 0;
+  if (CH == 4)
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      int t = chi(px[k], 3) - 255;
+      opaque_err += t * t;
+    }
+
+  if (s.mode_selection[0] != 0) {
+    enc_mode01237<0>(px, s, keys, stride, 16, true, opaque_err, best);
+    if (s.skip_mode2 == 0) enc_mode01237<2>(px, s, keys, stride, 64, true, opaque_err, best);
+  }
+  if (s.mode_selection[1] != 0) {
+    uint32_t t1 = s.fast_skip_threshold_mode1, t3 = s.fast_skip_threshold_mode3;
+    if (t1 != 0 || t3 != 0) {
+      partition_keys<3>(px, keys, stride);
+      select_smallest(keys, stride, (int)(t1 > t3 ? t1 : t3));
+      enc_mode01237<1>(px, s, keys, stride, (int)t1, false, opaque_err, best);
+      enc_mode01237<3>(px, s, keys, stride, (int)t3, false, opaque_err, best);
+    }
+    uint32_t t7 = s.fast_skip_threshold_mode7;
+    if (t7 != 0) {
+      partition_keys<CH>(px, keys, stride);
+      select_smallest(keys, stride, (int)t7);
+      enc_mode01237<7>(px, s, keys, stride, (int)t7, false, opaque_err, best);
+    }
+  }
+  if (s.mode_selection[2] != 0) {
+#pragma unroll 1
+    for (uint32_t p = s.mode45_channel0; p < s.channels; p++) {
+      enc_mode45_candidate<4, CH, 0>(px, s, p, best);
+      enc_mode45_candidate<4, CH, 1>(px, s, p, best);
+    }
+#pragma unroll 1
+    for (uint32_t p = s.mode45_channel0; p < s.channels; p++)
+      enc_mode45_candidate<5, CH, 0>(px, s, p, best);
+  }
+  if (s.mode_selection[3] != 0) enc_mode6<CH>(px, s, best);
+
+  if (best.mode == 8) {
+    // No candidate beat the initial error: the reference leaves its all-zero
+    // data words in place.
+    out[0] = out[1] = out[2] = out[3] = 0;
+  } else if (best.mode == 4 || best.mode == 5) {
+    code_mode45(best, out);
+  } else if (best.mode == 6) {
+    code_mode6(best, out);
+  } else {
+    code_mode01237(best, out);
+  }
+}
+
+}  // namespace bcn

@@ -1,0 +1,162 @@
+// Shared device helpers and format tables for the BCn kernels.
+//
+// Numeric contract (what makes the GPU output byte-identical to the reference
+// crate's CPU encoder, src/encode/*.rs):
+//   * the library is compiled with -ffp-contract=off and correctly rounded f32
+//     division and sqrt, so every float expression written here rounds exactly
+//     like the Rust source's;
+//   * sums of products of integer-valued floats whose every partial sum stays
+//     below 2^24 are exact in f32 in any order; those are computed in int32
+//     (each site states its bound);
+//   * Rust's saturating `as i32` followed by a clamp to [lo, hi] with lo >= 0
+//     equals trunc(min(max(x, lo), hi)) including NaN -> lo (qclamp below).
+//
+// BCN_HD marks functions that are compiled for gfx950 and, in the CPU
+// kernel-check build the tests use, for the host.
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define BCN_HD __host__ __device__ __forceinline__
+#define BCN_TABLE __constant__
+#else
+#include <cmath>
+#define BCN_HD inline
+#define BCN_TABLE static const
+#endif
+
+namespace bcn {
+
+// ---- numeric helpers --------------------------------------------------------
+
+BCN_HD float sq(float x) { return x * x; }
+
+// Rust `x as i32` then i32::clamp(lo, hi) for integer lo >= 0: NaN -> lo.
+BCN_HD int qclamp(float x, float lo, float hi) { return (int)fminf(fmaxf(x, lo), hi); }
+
+// Rust f32::clamp: `if x < lo {lo}; if x > hi {hi}` (NaN passes through).
+BCN_HD float rclamp(float x, float lo, float hi) {
+  x = x < lo ? lo : x;
+  return x > hi ? hi : x;
+}
+
+BCN_HD int iclamp(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+// A copy of x the compiler must treat as unknown: stops it from hoisting
+// per-channel texel extractions out of the candidate loops and keeping 48
+// unpacked values live; re-extracting a byte costs one instruction.
+BCN_HD uint32_t fresh(uint32_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm volatile("" : "+v"(x));
+#else
+  asm volatile("" : "+r"(x));
+#endif
+  return x;
+}
+
+// Byte p of a packed RGBA8 texel (a single v_cvt_f32_ubyteN / v_bfe on gfx950).
+BCN_HD float chf(uint32_t px, int p) { return (float)((px >> (8 * p)) & 0xFFu); }
+BCN_HD int chi(uint32_t px, int p) { return (int)((px >> (8 * p)) & 0xFFu); }
+
+// Index weights of BPTC interpolation (2/3/4-bit: 0,21,43,64 / 0,9,..,64 /
+// 0,4,..,64), one byte per index.
+BCN_HD int weight2(int i) { return (int)((0x402B1500u >> (8 * i)) & 0xFFu); }
+BCN_HD int weight3(int i) { return (int)((0x40372E251B120900ull >> (8 * i)) & 0xFFu); }
+BCN_HD int weight4(int i) {
+  const uint64_t lo = 0x1E1A15110D090400ull, hi = 0x403C37332F2B2622ull;
+  return (int)(((i < 8 ? lo : hi) >> (8 * (i & 7))) & 0xFFu);
+}
+template <int BITS>
+BCN_HD int weight(int i) {
+  return BITS == 2 ? weight2(i) : (BITS == 3 ? weight3(i) : weight4(i));
+}
+
+// ---- partition tables -------------------------------------------------------
+//
+// The BPTC partition shapes (2 subsets: 64 shapes, 3 subsets: 64 shapes) as
+// 2 bits of subset index per texel (texel k at bits 2k..2k+1) and the anchor
+// (fix-up) texels of subsets 1 and 2.  The encoder's per-subset texel masks
+// are derived from the shapes at compile time.
+struct PartitionTables {
+  uint32_t subset[128];  // [0,64): 2-subset shapes, [64,128): 3-subset shapes
+  uint32_t mask01[128];  // texel mask of subset 0 (low 16 bits) and subset 1 (high 16)
+  uint8_t anchor1[128];  // anchor texel of subset 1
+  uint8_t anchor2[128];  // anchor texel of subset 2 (3-subset shapes; 0 otherwise)
+};
+
+constexpr PartitionTables make_partition_tables() {
+  constexpr uint32_t shapes[128] = {
+      0x50505050u, 0x40404040u, 0x54545454u, 0x54505040u, 0x50404000u, 0x55545450u, 0x55545040u,
+      0x54504000u, 0x50400000u, 0x55555450u, 0x55544000u, 0x54400000u, 0x55555440u, 0x55550000u,
+      0x55555500u, 0x55000000u, 0x55150100u, 0x00004054u, 0x15010000u, 0x00405054u, 0x00004050u,
+      0x15050100u, 0x05010000u, 0x40505054u, 0x00404050u, 0x05010100u, 0x14141414u, 0x05141450u,
+      0x01155440u, 0x00555500u, 0x15014054u, 0x05414150u, 0x44444444u, 0x55005500u, 0x11441144u,
+      0x05055050u, 0x05500550u, 0x11114444u, 0x41144114u, 0x44111144u, 0x15055054u, 0x01055040u,
+      0x05041050u, 0x05455150u, 0x14414114u, 0x50050550u, 0x41411414u, 0x00141400u, 0x00041504u,
+      0x00105410u, 0x10541000u, 0x04150400u, 0x50410514u, 0x41051450u, 0x05415014u, 0x14054150u,
+      0x41050514u, 0x41505014u, 0x40011554u, 0x54150140u, 0x50505500u, 0x00555050u, 0x15151010u,
+      0x54540404u, 0xAA685050u, 0x6A5A5040u, 0x5A5A4200u, 0x5450A0A8u, 0xA5A50000u, 0xA0A05050u,
+      0x5555A0A0u, 0x5A5A5050u, 0xAA550000u, 0xAA555500u, 0xAAAA5500u, 0x90909090u, 0x94949494u,
+      0xA4A4A4A4u, 0xA9A59450u, 0x2A0A4250u, 0xA5945040u, 0x0A425054u, 0xA5A5A500u, 0x55A0A0A0u,
+      0xA8A85454u, 0x6A6A4040u, 0xA4A45000u, 0x1A1A0500u, 0x0050A4A4u, 0xAAA59090u, 0x14696914u,
+      0x69691400u, 0xA08585A0u, 0xAA821414u, 0x50A4A450u, 0x6A5A0200u, 0xA9A58000u, 0x5090A0A8u,
+      0xA8A09050u, 0x24242424u, 0x00AA5500u, 0x24924924u, 0x24499224u, 0x50A50A50u, 0x500AA550u,
+      0xAAAA4444u, 0x66660000u, 0xA5A0A5A0u, 0x50A050A0u, 0x69286928u, 0x44AAAA44u, 0x66666600u,
+      0xAA444444u, 0x54A854A8u, 0x95809580u, 0x96969600u, 0xA85454A8u, 0x80959580u, 0xAA141414u,
+      0x96960000u, 0xAAAA1414u, 0xA05050A0u, 0xA0A5A5A0u, 0x96000000u, 0x40804080u, 0xA9A8A9A8u,
+      0xAAAAAA44u, 0x2A4A5254u};
+  constexpr uint8_t a1[128] = {
+      15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 2,  8,  2,  2,  8,
+      8,  15, 2,  8,  2,  2,  8,  8,  2,  2,  15, 15, 6,  8,  2,  8,  15, 15, 2,  8,  2,  2,
+      2,  15, 15, 6,  6,  2,  6,  8,  15, 15, 2,  2,  15, 15, 15, 15, 15, 2,  2,  15, 3,  3,
+      15, 15, 8,  3,  15, 15, 8,  8,  6,  6,  6,  5,  3,  3,  3,  3,  8,  15, 3,  3,  6,  10,
+      5,  8,  8,  6,  8,  5,  15, 15, 8,  15, 3,  5,  6,  10, 8,  15, 15, 3,  15, 5,  15, 15,
+      15, 15, 3,  15, 5,  5,  5,  8,  5,  10, 5,  10, 8,  13, 15, 12, 3,  3};
+  constexpr uint8_t a2[64] = {15, 8,  8,  3,  15, 15, 3,  8,  15, 15, 15, 15, 15, 15, 15, 8,
+                              15, 8,  15, 3,  15, 8,  15, 8,  3,  15, 6,  10, 15, 15, 10, 8,
+                              15, 3,  15, 10, 10, 8,  9,  10, 6,  15, 8,  15, 3,  6,  6,  8,
+                              15, 3,  15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 3,  15, 15, 8};
+  PartitionTables t{};
+  for (int i = 0; i < 128; i++) {
+    t.subset[i] = shapes[i];
+    uint32_t m0 = 0, m1 = 0;
+    for (int k = 0; k < 16; k++) {
+      uint32_t s = (shapes[i] >> (2 * k)) & 3u;
+      if (s == 0) m0 |= 1u << k;
+      if (s == 1) m1 |= 1u << k;
+    }
+    t.mask01[i] = m0 | (m1 << 16);
+    t.anchor1[i] = a1[i];
+    t.anchor2[i] = i < 64 ? 0 : a2[i - 64];
+  }
+  return t;
+}
+
+BCN_TABLE PartitionTables kPart = make_partition_tables();
+
+// Texel mask (16 bits) of subset j of partition `part`.
+BCN_HD uint32_t part_mask(int part, int j) {
+  uint32_t m = kPart.mask01[part];
+  uint32_t m0 = m & 0xFFFFu, m1 = m >> 16;
+  return j == 0 ? m0 : (j == 1 ? m1 : (~(m0 | m1) & 0xFFFFu));
+}
+
+// ---- 128-bit little-endian bit writer ----------------------------------------
+
+struct BitWriter {
+  uint64_t lo = 0, hi = 0;
+  uint32_t pos = 0;
+  BCN_HD void put(uint32_t nbits, uint32_t v) {
+    uint64_t x = (uint64_t)v;
+    if (pos < 64) {
+      lo |= x << pos;
+      if (pos + nbits > 64) hi |= x >> (64 - pos);
+    } else {
+      hi |= x << (pos - 64);
+    }
+    pos += nbits;
+  }
+};
+
+}  // namespace bcn

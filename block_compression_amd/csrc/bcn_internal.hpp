@@ -1,0 +1,28 @@
+// Host-side task description shared by the C ABI and the kernel launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/bcn_mi355x.h"
+
+namespace bcn {
+
+struct Task {
+  bcn_variant variant;
+  uint32_t settings[19];          // bcn_bc7_settings or bcn_bc6h_settings words
+  const uint8_t* texture;         // texel (0, texture_y_offset)
+  bcn_source_format format;
+  uint32_t pitch;                 // bytes per texture row
+  uint32_t width, height;         // region size (multiples of 4)
+  uint8_t* blocks;                // first block (buffer + blocks_offset rounded to words)
+};
+
+// Blocks per row / column of a task.
+inline uint32_t blocks_x(const Task& t) { return (t.width + 3) / 4; }
+inline uint32_t blocks_y(const Task& t) { return (t.height + 3) / 4; }
+
+hipError_t launch_bc7(const Task& t, hipStream_t stream);
+hipError_t launch_bc15(const Task& t, hipStream_t stream);
+hipError_t launch_bc6h(const Task& t, hipStream_t stream);
+
+}  // namespace bcn

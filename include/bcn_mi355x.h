@@ -1,0 +1,155 @@
+/*
+ * bcn_mi355x -- BC1-BC7 block compression on AMD MI355X (gfx950).
+ *
+ * C ABI of the GPU block compressor.  It replaces the reference crate's
+ * `GpuBlockCompressor` (block_compression v0.3.0, src/block_compressor.rs)
+ * and its WGSL kernels (src/shader/ bc1_to_5, bc6h, bc7 .wgsl): the same task API -- create,
+ * add tasks, compress -- over the same formats and encoder presets, with
+ * device pointers and a HIP stream where the reference takes wgpu textures,
+ * buffers and a compute pass.  Every `assert!` of the reference becomes a
+ * status code.  The encoded bytes equal the crate's CPU encoder output
+ * (src/encode.rs) for the same input.
+ *
+ * Threading: one compressor per host thread (the reference's methods take
+ * `&mut self`).  Work is asynchronous on the stream given to bcn_compress.
+ */
+#ifndef BCN_MI355X_H
+#define BCN_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* CompressionVariant (src/lib.rs:74-103), in declaration order. */
+typedef enum bcn_variant {
+  BCN_BC1 = 0,  /* RGB, 8 bytes per block */
+  BCN_BC2 = 1,  /* RGBA with 4-bit alpha, 16 bytes */
+  BCN_BC3 = 2,  /* RGBA with interpolated alpha, 16 bytes */
+  BCN_BC4 = 3,  /* R, 8 bytes */
+  BCN_BC5 = 4,  /* RG, 16 bytes */
+  BCN_BC6H = 5, /* RGB half float (unsigned), 16 bytes; settings: bcn_bc6h_settings */
+  BCN_BC7 = 6   /* RGB(A), 16 bytes; settings: bcn_bc7_settings */
+} bcn_variant;
+
+/* Source texel formats (the reference's texture views: Rgba8Unorm and
+ * Rgba16Float, src/block_compressor.rs:301-313). */
+typedef enum bcn_source_format {
+  BCN_SRC_RGBA8 = 0,   /* 4 x u8 per texel, read as exact integers */
+  BCN_SRC_RGBA16F = 1  /* 4 x IEEE binary16 per texel (BC6H only) */
+} bcn_source_format;
+
+typedef enum bcn_status {
+  BCN_OK = 0,
+  BCN_ERR_INVALID_ARGUMENT = 1,     /* null handle/pointer, bad enum, bad settings size */
+  BCN_ERR_NOT_MULTIPLE_OF_4 = 2,    /* width, height or texture_y_offset (block_compressor.rs:352-357) */
+  BCN_ERR_BUFFER_TOO_SMALL = 3,     /* block_compressor.rs:364-373 */
+  BCN_ERR_TEXTURE_TOO_SMALL = 4,    /* rows y_offset..y_offset+height exceed texture_height */
+  BCN_ERR_UNSUPPORTED = 5,          /* e.g. RGBA16F source for a non-BC6H variant */
+  BCN_ERR_MISALIGNED = 6,           /* source/destination not 4-byte aligned */
+  BCN_ERR_HIP = 7,                  /* a HIP runtime call failed; see bcn_last_error */
+  BCN_ERR_NO_DEVICE = 8
+} bcn_status;
+
+/* BC6HSettings, 5 x u32 repr(C) (src/settings.rs:9-15). */
+typedef struct bcn_bc6h_settings {
+  uint32_t slow_mode;
+  uint32_t fast_mode;
+  uint32_t refine_iterations_1p;
+  uint32_t refine_iterations_2p;
+  uint32_t fast_skip_threshold;
+} bcn_bc6h_settings;
+
+/* BC7Settings, 19 x u32 repr(C) (src/settings.rs:80-90). */
+typedef struct bcn_bc7_settings {
+  uint32_t refine_iterations[8];
+  uint32_t mode_selection[4];
+  uint32_t skip_mode2;
+  uint32_t fast_skip_threshold_mode1;
+  uint32_t fast_skip_threshold_mode3;
+  uint32_t fast_skip_threshold_mode7;
+  uint32_t mode45_channel0;
+  uint32_t refine_iterations_channel;
+  uint32_t channels;
+} bcn_bc7_settings;
+
+typedef struct bcn_compressor bcn_compressor;
+
+/* GpuBlockCompressor::new (src/block_compressor.rs:66).  `device` is a HIP
+ * device ordinal. */
+int bcn_compressor_create(int device, bcn_compressor** out);
+
+/* Drop of GpuBlockCompressor. */
+void bcn_compressor_destroy(bcn_compressor* c);
+
+/* GpuBlockCompressor::add_compression_task (src/block_compressor.rs:342-387).
+ *
+ * texture:          device pointer to texel (0, 0) of the source texture.
+ * texture_pitch:    bytes per texture row.
+ * texture_height:   rows in the texture (bounds check for the y offset).
+ * width, height:    size of the region to compress; multiples of 4.
+ * buffer:           device pointer to the destination buffer.
+ * buffer_size:      its size in bytes.
+ * texture_y_offset: first texture row of the region; a multiple of 4.  It is
+ *                   honoured for every format (the reference's WGSL ignores
+ *                   it for BC1/BC3/BC4/BC5, bc1_to_5.wgsl:44-72).
+ * blocks_offset:    byte offset of the first block in `buffer`; like the
+ *                   reference (block_compressor.rs:449) it is used in whole
+ *                   32-bit words, i.e. rounded down to a multiple of 4.
+ * settings:         bcn_bc6h_settings for BC6H, bcn_bc7_settings for BC7,
+ *                   otherwise NULL; settings_size is its sizeof.
+ *
+ * Blocks are written row-major, block (x, y) at
+ * blocks_offset + (y * ceil(width/4) + x) * block_bytes. */
+int bcn_add_compression_task(bcn_compressor* c, bcn_variant variant, const void* settings,
+                             size_t settings_size, const void* texture,
+                             bcn_source_format texture_format, uint32_t texture_pitch,
+                             uint32_t texture_height, uint32_t width, uint32_t height,
+                             void* buffer, size_t buffer_size, uint32_t texture_y_offset,
+                             uint32_t blocks_offset);
+
+/* GpuBlockCompressor::compress (src/block_compressor.rs:540-590): enqueue
+ * every queued task on `hip_stream` (a hipStream_t; NULL = default stream)
+ * and empty the queue.  Asynchronous: synchronise the stream before reading
+ * the buffers. */
+int bcn_compress(bcn_compressor* c, void* hip_stream);
+
+/* Number of queued tasks. */
+size_t bcn_task_count(const bcn_compressor* c);
+
+/* Text of the last failure on this compressor (empty if none). */
+const char* bcn_last_error(const bcn_compressor* c);
+
+/* Name of a status code. */
+const char* bcn_status_string(int status);
+
+/* CompressionVariant::blocks_byte_size (src/lib.rs:138-144). */
+size_t bcn_blocks_byte_size(bcn_variant variant, uint32_t width, uint32_t height);
+
+/* GPU decode of a block image into RGBA8, the layout of
+ * decompress_blocks_as_rgba8 (src/decode.rs:336-387): BC4 -> (R,0,0,0),
+ * BC5 -> (R,G,0,0); BC6H is not accepted here (its RGBA8 path needs powf).
+ * blocks: device pointer, ceil(w/4)*ceil(h/4) blocks; rgba: device pointer,
+ * rgba_pitch bytes per row; width and height multiples of 4. */
+int bcn_decode_rgba8(bcn_variant variant, const void* blocks, size_t blocks_size, uint32_t width,
+                     uint32_t height, void* rgba, uint32_t rgba_pitch, void* hip_stream);
+
+/* GPU decode of BC6H blocks into RGBA16F (decode_block_bc6h, src/decode/block.rs:262-769,
+ * unsigned unless is_signed) with alpha written as 1.0 (0x3C00).  rgba_pitch in bytes. */
+int bcn_decode_bc6h_rgba16f(const void* blocks, size_t blocks_size, uint32_t width,
+                            uint32_t height, int is_signed, void* rgba, uint32_t rgba_pitch,
+                            void* hip_stream);
+
+/* Copies of the encoder's partition tables (pattern: 2 bits of subset per
+ * texel; mask01: subset-0 mask | subset-1 mask << 16; anchors) for table
+ * checks without a GPU.  Each array has 128 entries. */
+void bcn_partition_tables(uint32_t* pattern, uint32_t* mask01, uint8_t* anchor1,
+                          uint8_t* anchor2);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BCN_MI355X_H */

@@ -1,0 +1,216 @@
+"""ctypes bridge to the CPU oracle (oracle/_build/liboracle_bc.so).
+
+Test infrastructure: the oracle is the checker the GPU path is compared
+against, and the bench's CPU baseline.  Nothing in block_compression_amd/
+imports this module.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+LIB_PATH = os.path.join(ORACLE_DIR, "_build", "liboracle_bc.so")
+OPCOUNT_PATH = os.path.join(ORACLE_DIR, "_build", "liboracle_bc_opcount.so")
+
+_lib = None
+_oplib = None
+
+_vp = ctypes.c_void_p
+_sz = ctypes.c_size_t
+_u32 = ctypes.c_uint32
+_i32 = ctypes.c_int
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def _declare(lib):
+    lib.oracle_compress_rgba8.argtypes = [_i32, _vp, _vp, _vp, _sz, _u32, _u32, _u32, _i32]
+    lib.oracle_compress_rgba8.restype = _i32
+    lib.oracle_compress_rgba16.argtypes = [_vp, _vp, _vp, _sz, _u32, _u32, _u32, _i32]
+    lib.oracle_compress_rgba16.restype = _i32
+    lib.oracle_compress_rgba8_sample.argtypes = [_i32, _vp, _vp, _u32, _u32, _vp, _sz, _vp, _i32]
+    lib.oracle_compress_rgba8_sample.restype = _i32
+    lib.oracle_bc7_block_errors.argtypes = [_vp, _vp, _u32, _u32, _u32, _vp, _vp]
+    lib.oracle_bc6h_block_errors.argtypes = [_vp, _vp, _u32, _u32, _u32, _vp, _vp]
+    for n in ("bc1", "bc2", "bc3", "bc4", "bc5", "bc7"):
+        getattr(lib, f"oracle_decode_block_{n}").argtypes = [_vp, _vp, _sz]
+        getattr(lib, f"oracle_decode_block_{n}").restype = None
+    lib.oracle_decode_block_bc6h.argtypes = [_vp, _vp, _sz, _i32]
+    lib.oracle_decode_block_bc6h_float.argtypes = [_vp, _vp, _sz, _i32]
+    lib.oracle_decompress_rgba8.argtypes = [_i32, _u32, _u32, _vp, _sz, _vp, _sz]
+    lib.oracle_decompress_rgba8.restype = _i32
+    lib.oracle_partial_sort_list.argtypes = [_vp, _sz, _u32]
+    lib.oracle_get_tables.argtypes = [_vp, _vp, _vp, _vp, _vp]
+    lib.oracle_get_partition_sets.argtypes = [_vp]
+    lib.oracle_f32_to_f16.argtypes = [ctypes.c_float]
+    lib.oracle_f32_to_f16.restype = ctypes.c_uint16
+    lib.oracle_f16_to_f32.argtypes = [ctypes.c_uint16]
+    lib.oracle_f16_to_f32.restype = ctypes.c_float
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def oplib():
+    global _oplib
+    if _oplib is None:
+        if not os.path.exists(OPCOUNT_PATH):
+            build()
+        _oplib = ctypes.CDLL(OPCOUNT_PATH)
+        _declare(_oplib)
+        _oplib.oracle_count_ops_rgba8.argtypes = [_i32, _vp, _vp, _u32, _u32, _vp, _sz]
+        _oplib.oracle_count_ops_rgba8.restype = ctypes.c_uint64
+        _oplib.oracle_count_ops_rgba16.argtypes = [_vp, _vp, _u32, _u32, _vp, _sz]
+        _oplib.oracle_count_ops_rgba16.restype = ctypes.c_uint64
+    return _oplib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def block_bytes(tag: int) -> int:
+    return 8 if tag in (0, 3) else 16
+
+
+def compress_rgba8(tag: int, settings: bytes, rgba: np.ndarray, threads: int = 1) -> np.ndarray:
+    """compress_rgba8 (encode.rs:83-151) over a (h, w, 4) uint8 image."""
+    rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
+    h, w, _ = rgba.shape
+    out = np.zeros(((w + 3) // 4) * ((h + 3) // 4) * block_bytes(tag), np.uint8)
+    sbuf = ctypes.create_string_buffer(settings or b"\0" * 4)
+    rc = lib().oracle_compress_rgba8(tag, sbuf, _ptr(rgba), _ptr(out), out.size, w, h, w * 4, threads)
+    if rc != 0:
+        raise ValueError(f"oracle_compress_rgba8 failed: {rc}")
+    return out
+
+
+def compress_rgba16(settings: bytes, rgba16: np.ndarray, threads: int = 1) -> np.ndarray:
+    """compress_rgba16 (encode.rs:211-251) over a (h, w, 4) float16 image."""
+    bits = np.ascontiguousarray(rgba16).view(np.uint16)
+    h, w, _ = bits.shape
+    out = np.zeros((w // 4) * (h // 4) * 16, np.uint8)
+    sbuf = ctypes.create_string_buffer(settings)
+    rc = lib().oracle_compress_rgba16(sbuf, _ptr(bits), _ptr(out), out.size, w, h, w * 4, threads)
+    if rc != 0:
+        raise ValueError(f"oracle_compress_rgba16 failed: {rc}")
+    return out
+
+
+def compress_rgba8_sample(tag: int, settings: bytes, rgba: np.ndarray, block_ids: np.ndarray,
+                          threads: int = 1) -> np.ndarray:
+    rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
+    h, w, _ = rgba.shape
+    ids = np.ascontiguousarray(block_ids, dtype=np.int64)
+    out = np.zeros(ids.size * block_bytes(tag), np.uint8)
+    sbuf = ctypes.create_string_buffer(settings or b"\0" * 4)
+    lib().oracle_compress_rgba8_sample(tag, sbuf, _ptr(rgba), w, w * 4, _ptr(ids), ids.size,
+                                       _ptr(out), threads)
+    return out
+
+
+def bc7_block_errors(settings: bytes, rgba: np.ndarray):
+    rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
+    h, w, _ = rgba.shape
+    n = (w // 4) * (h // 4)
+    out = np.zeros(n * 16, np.uint8)
+    errs = np.zeros(n, np.float32)
+    sbuf = ctypes.create_string_buffer(settings)
+    lib().oracle_bc7_block_errors(sbuf, _ptr(rgba), w, h, w * 4, _ptr(out), _ptr(errs))
+    return out, errs
+
+
+def bc6h_block_errors(settings: bytes, rgba16: np.ndarray):
+    bits = np.ascontiguousarray(rgba16).view(np.uint16)
+    h, w, _ = bits.shape
+    n = (w // 4) * (h // 4)
+    out = np.zeros(n * 16, np.uint8)
+    errs = np.zeros(n, np.float32)
+    sbuf = ctypes.create_string_buffer(settings)
+    lib().oracle_bc6h_block_errors(sbuf, _ptr(bits), w, h, w * 4, _ptr(out), _ptr(errs))
+    return out, errs
+
+
+def decode_block(name: str, block: bytes, pitch: int, out_len: int) -> np.ndarray:
+    """One block through decode_block_<name> with the given pitch (block.rs)."""
+    src = np.frombuffer(bytes(block), np.uint8).copy()
+    out = np.zeros(out_len, np.uint8)
+    getattr(lib(), f"oracle_decode_block_{name}")(_ptr(src), _ptr(out), pitch)
+    return out
+
+
+def decode_block_bc6h(block: bytes, pitch: int, is_signed: bool) -> np.ndarray:
+    src = np.frombuffer(bytes(block), np.uint8).copy()
+    out = np.zeros(4 * pitch, np.uint16)
+    lib().oracle_decode_block_bc6h(_ptr(src), _ptr(out), pitch, int(is_signed))
+    return out
+
+
+def decode_block_bc6h_float(block: bytes, pitch: int, is_signed: bool) -> np.ndarray:
+    src = np.frombuffer(bytes(block), np.uint8).copy()
+    out = np.zeros(4 * pitch, np.float32)
+    lib().oracle_decode_block_bc6h_float(_ptr(src), _ptr(out), pitch, int(is_signed))
+    return out
+
+
+def decompress_rgba8(tag: int, width: int, height: int, blocks: np.ndarray) -> np.ndarray:
+    """decompress_blocks_as_rgba8 (decode.rs:336-387)."""
+    blocks = np.ascontiguousarray(blocks, dtype=np.uint8)
+    out = np.zeros(width * height * 4, np.uint8)
+    rc = lib().oracle_decompress_rgba8(tag, width, height, _ptr(blocks), blocks.size, _ptr(out), out.size)
+    if rc != 0:
+        raise ValueError(f"oracle_decompress_rgba8 failed: {rc}")
+    return out.reshape(height, width, 4)
+
+
+def tables():
+    n = 128
+    pattern = np.zeros(n, np.uint32)
+    m0 = np.zeros(n, np.uint32)
+    m1 = np.zeros(n, np.uint32)
+    m2 = np.zeros(n, np.uint32)
+    skips = np.zeros(n * 3, np.uint32)
+    lib().oracle_get_tables(_ptr(pattern), _ptr(m0), _ptr(m1), _ptr(m2), _ptr(skips))
+    return pattern, m0, m1, m2, skips.reshape(n, 3)
+
+
+def partition_sets() -> np.ndarray:
+    out = np.zeros((2, 64, 16), np.uint8)
+    lib().oracle_get_partition_sets(_ptr(out))
+    return out
+
+
+def partial_sort_list(values, partial_count: int) -> np.ndarray:
+    a = np.ascontiguousarray(values, dtype=np.int32).copy()
+    lib().oracle_partial_sort_list(_ptr(a), a.size, partial_count)
+    return a
+
+
+def count_ops_rgba8(tag: int, settings: bytes, rgba: np.ndarray, block_ids: np.ndarray) -> int:
+    rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
+    h, w, _ = rgba.shape
+    ids = np.ascontiguousarray(block_ids, dtype=np.int64)
+    sbuf = ctypes.create_string_buffer(settings or b"\0" * 4)
+    return int(oplib().oracle_count_ops_rgba8(tag, sbuf, _ptr(rgba), w, w * 4, _ptr(ids), ids.size))
+
+
+def count_ops_rgba16(settings: bytes, rgba16: np.ndarray, block_ids: np.ndarray) -> int:
+    bits = np.ascontiguousarray(rgba16).view(np.uint16)
+    h, w, _ = bits.shape
+    ids = np.ascontiguousarray(block_ids, dtype=np.int64)
+    sbuf = ctypes.create_string_buffer(settings)
+    return int(oplib().oracle_count_ops_rgba16(sbuf, _ptr(bits), w, w * 4, _ptr(ids), ids.size))
