@@ -63,7 +63,22 @@ def main() -> int:
     ap.add_argument("--parity-blocks", type=int, default=65536,
                     help="seeded sample of blocks checked against the CPU oracle (0 = none)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0 = min(16, cores))")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB five times "
+                         "(reference kernel for scaling HBM byte counters) and exit")
     args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+
+        n = args.calibrate_copy << 20
+        a = torch.empty(n, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": n, "copies": 5}), flush=True)
+        return 0
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the path is single-GPU "

@@ -100,6 +100,11 @@ int bcn_add_compression_task(bcn_compressor* c, bcn_variant variant, const void*
     return fail(c, BCN_ERR_INVALID_ARGUMENT, "unknown texture format");
   if (texture_format == BCN_SRC_RGBA16F && variant != BCN_BC6H)
     return fail(c, BCN_ERR_UNSUPPORTED, "an RGBA16F source is only accepted for BC6H");
+  // The reference's RGBA8 route into BC6H (compress_rgba8, encode.rs:130-137)
+  // converts through powf-based sRGB decoding; the GPU path takes half-float
+  // texels only (compress_rgba16, encode.rs:211-251).
+  if (texture_format == BCN_SRC_RGBA8 && variant == BCN_BC6H)
+    return fail(c, BCN_ERR_UNSUPPORTED, "BC6H takes an RGBA16F source");
   bcn::Task t{};
   t.variant = variant;
   if (variant == BCN_BC7) {

@@ -58,3 +58,71 @@ extern "C" int kcheck_bc15_encode(int fmt, const uint8_t* rgba, uint32_t width, 
   }
   return 0;
 }
+
+#include "../../block_compression_amd/csrc/bc6h_block.hpp"
+
+// BC6H through the same per-block function as the HIP kernel.  `rgba16` are
+// f16 bit patterns, stride in f16 elements.
+extern "C" int kcheck_bc6h_encode(const void* settings, const uint16_t* rgba16, uint32_t width,
+                                  uint32_t height, uint32_t stride, uint8_t* out) {
+  bcn::Bc6hParams s;
+  std::memcpy(&s, settings, sizeof(s));
+  uint32_t bw = width / 4, bh = height / 4;
+  for (uint32_t idx = 0; idx < bw * bh; idx++) {
+    uint32_t yy = idx / bw, xx = idx % bw;
+    uint16_t bits[48];
+    for (int y = 0; y < 4; y++)
+      for (int x = 0; x < 4; x++) {
+        const uint16_t* p = rgba16 + (size_t)(yy * 4 + y) * stride + (xx * 4 + x) * 4;
+        for (int c = 0; c < 3; c++) bits[c * 16 + y * 4 + x] = p[c];
+      }
+    int keys[32];
+    uint32_t o[4];
+    bcn::encode_bc6h_block(bits, s, keys, 1, o);
+    std::memcpy(out + (size_t)idx * 16, o, 16);
+  }
+  return 0;
+}
+
+#include "../../block_compression_amd/csrc/decode_block.hpp"
+
+// Decoders: `count` blocks of `bb` bytes -> 16 texels each.  fmt 0..4, 6
+// write RGBA8 texels (BC4 -> R,0,0,0; BC5 -> R,G,0,0); fmt 5 writes 48 f16
+// bit patterns (RGB per texel).
+extern "C" int kcheck_decode(int fmt, int is_signed, const uint8_t* blocks, size_t count, void* out) {
+  size_t bb = (fmt == 0 || fmt == 3) ? 8 : 16;
+  for (size_t i = 0; i < count; i++) {
+    uint32_t in[4] = {0, 0, 0, 0};
+    std::memcpy(in, blocks + i * bb, bb);
+    if (fmt == 5) {
+      uint16_t h[48];
+      bcn::decode_bc6h(in, is_signed != 0, h);
+      std::memcpy((uint16_t*)out + i * 48, h, sizeof(h));
+      continue;
+    }
+    uint32_t px[16];
+    if (fmt == 0) bcn::decode_color(in[0], in[1], false, px);
+    else if (fmt == 1 || fmt == 2) {
+      bcn::decode_color(in[2], in[3], true, px);
+      uint8_t a[16];
+      if (fmt == 1)
+        for (int k = 0; k < 16; k++) a[k] = (uint8_t)((((k < 8 ? in[0] : in[1]) >> (4 * (k & 7))) & 15u) * 17u);
+      else
+        bcn::decode_channel(in[0], in[1], a);
+      for (int k = 0; k < 16; k++) px[k] = (px[k] & 0x00FFFFFFu) | ((uint32_t)a[k] << 24);
+    } else if (fmt == 3) {
+      uint8_t r[16];
+      bcn::decode_channel(in[0], in[1], r);
+      for (int k = 0; k < 16; k++) px[k] = r[k];
+    } else if (fmt == 4) {
+      uint8_t r[16], g[16];
+      bcn::decode_channel(in[0], in[1], r);
+      bcn::decode_channel(in[2], in[3], g);
+      for (int k = 0; k < 16; k++) px[k] = r[k] | ((uint32_t)g[k] << 8);
+    } else {
+      bcn::decode_bc7(in, px);
+    }
+    std::memcpy((uint32_t*)out + i * 16, px, sizeof(px));
+  }
+  return 0;
+}

@@ -1,0 +1,59 @@
+"""ctypes bridge to tests/kernel_check/_build/libkernel_check.so: the
+product's per-block kernel code (block_compression_amd/csrc/*_block.hpp)
+compiled for the host, so its arithmetic can be compared with the oracle
+without a GPU."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "kernel_check", "_build", "libkernel_check.so")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        subprocess.run([os.path.join(HERE, "kernel_check", "build.sh")], check=True)
+        _lib = ctypes.CDLL(LIB)
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def bc7_encode(settings: bytes, img: np.ndarray) -> np.ndarray:
+    img = np.ascontiguousarray(img)
+    h, w, _ = img.shape
+    out = np.zeros((w // 4) * (h // 4) * 16, np.uint8)
+    lib().kcheck_bc7_encode(ctypes.create_string_buffer(settings), _p(img), w, h, w * 4, _p(out))
+    return out
+
+
+def bc15_encode(fmt: int, img: np.ndarray) -> np.ndarray:
+    img = np.ascontiguousarray(img)
+    h, w, _ = img.shape
+    bb = 8 if fmt in (0, 3) else 16
+    out = np.zeros((w // 4) * (h // 4) * bb, np.uint8)
+    lib().kcheck_bc15_encode(fmt, _p(img), w, h, w * 4, _p(out))
+    return out
+
+
+def bc6h_encode(settings: bytes, img16: np.ndarray) -> np.ndarray:
+    bits = np.ascontiguousarray(img16).view(np.uint16)
+    h, w, _ = bits.shape
+    out = np.zeros((w // 4) * (h // 4) * 16, np.uint8)
+    lib().kcheck_bc6h_encode(ctypes.create_string_buffer(settings), _p(bits), w, h, w * 4, _p(out))
+    return out
+
+
+def decode(fmt: int, blocks: np.ndarray, is_signed: bool = False) -> np.ndarray:
+    blocks = np.ascontiguousarray(blocks, dtype=np.uint8)
+    bb = 8 if fmt in (0, 3) else 16
+    n = blocks.size // bb
+    out = np.zeros(n * 48, np.uint16) if fmt == 5 else np.zeros(n * 16, np.uint32)
+    lib().kcheck_decode(fmt, int(is_signed), _p(blocks), n, _p(out))
+    return out
