@@ -42,9 +42,13 @@ struct Bc7Params {
 
 // Current winner.  qep holds endpoint e (0..5) channel p (0..3) at byte e*4+p
 // of word e; for modes 4/5 the scalar channel's two endpoints sit in the
-// channel-3 bytes of endpoints 0 and 1.
+// channel-3 bytes of endpoints 0 and 1.  A candidate replaces the winner if
+// its error is smaller, or equal with an earlier position in the reference's
+// search order (bc7.rs:853-867, :740-753): that is the reference's strict-<,
+// first-wins rule, independent of the order the kernel evaluates candidates.
 struct Bc7Best {
   int err;
+  int order;  // position of the candidate in the reference's search order
   uint32_t mode, part, rotation, swap;
   uint32_t qep[6];
   uint32_t qblock[2];
@@ -97,13 +101,13 @@ struct Moments {
   int n, s[4], q[10];
 };
 
-template <int CH>
-BCN_HD Moments moments(const uint32_t px[16], uint32_t mask) {
+template <int CH, int ROT = 0>
+BCN_HD Moments moments(const uint32_t px[16], uint32_t mask, uint32_t sh = 0) {
   Moments m{};
 #pragma unroll
   for (int k = 0; k < 16; k++) {
     int f = (int)((mask >> k) & 1u);
-    uint32_t t = fresh(px[k]);
+    uint32_t t = texel<ROT>(px, k, sh);
     int v0 = f ? chi(t, 0) : 0, v1 = f ? chi(t, 1) : 0, v2 = f ? chi(t, 2) : 0;
     m.n += f;
     m.s[0] += v0;
@@ -150,9 +154,10 @@ BCN_HD void covar_from_moments(float c[10], const int s[4], const int q[10], int
 
 // block_segment_core (common.rs:483-518): principal-axis endpoints of the
 // masked texels, before the BC7 clamp.
-template <int CH>
-BCN_HD void pca_segment(const uint32_t px[16], uint32_t mask, float e0[4], float e1[4]) {
-  Moments m = moments<CH>(px, mask);
+template <int CH, int ROT = 0>
+BCN_HD void pca_segment(const uint32_t px[16], uint32_t mask, float e0[4], float e1[4],
+                        uint32_t sh = 0) {
+  Moments m = moments<CH, ROT>(px, mask, sh);
   float s14 = (float)m.n;
   float dc[4];
 #pragma unroll
@@ -176,7 +181,7 @@ BCN_HD void pca_segment(const uint32_t px[16], uint32_t mask, float e0[4], float
     // The reference's running sum starts at 0.0; dropping that leading add
     // can only turn a +0.0 into -0.0, and the sign of a zero projection never
     // reaches an output (ext*axis + dc rounds it away).
-    uint32_t t = fresh(px[k]);
+    uint32_t t = texel<ROT>(px, k, sh);
     float dot = axis[0] * (chf(t, 0) - dc[0]);
     dot = dot + axis[1] * (chf(t, 1) - dc[1]);
     dot = dot + axis[2] * (chf(t, 2) - dc[2]);
@@ -197,9 +202,10 @@ BCN_HD void pca_segment(const uint32_t px[16], uint32_t mask, float e0[4], float
 }
 
 // block_segment (bc7.rs:371-379): PCA endpoints clamped to [0, 255].
-template <int CH>
-BCN_HD void bc7_segment(const uint32_t px[16], uint32_t mask, float e0[4], float e1[4]) {
-  pca_segment<CH>(px, mask, e0, e1);
+template <int CH, int ROT = 0>
+BCN_HD void bc7_segment(const uint32_t px[16], uint32_t mask, float e0[4], float e1[4],
+                        uint32_t sh = 0) {
+  pca_segment<CH, ROT>(px, mask, e0, e1, sh);
 #pragma unroll
   for (int p = 0; p < CH; p++) {
     e0[p] = rclamp(e0[p], 0.0f, 255.0f);
@@ -321,9 +327,9 @@ BCN_HD void quant_pair(const float e0[4], const float e1[4], int q0[4], int q1[4
 // Index per texel against dequantised (integer) endpoints d[j][0|1][p] of
 // subset j, pattern giving each texel's subset.  Returns the exact SSE.
 // Bounds: |proj|, div <= 4*255^2 and the error sum <= 16*4*255^2 < 2^24.
-template <int CH, int BITS, int NSUB>
+template <int CH, int BITS, int NSUB, int ROT = 0>
 BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pattern,
-                       uint32_t qblock[2]) {
+                       uint32_t qblock[2], uint32_t sh = 0) {
   constexpr int L = 1 << BITS;
   int dd[3][4] = {};
   float fdiv[3] = {1.0f, 1.0f, 1.0f};
@@ -351,7 +357,7 @@ BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pat
       de[p] = NSUB == 1 ? dd[0][p] : (j == 0 ? dd[0][p] : (j == 1 ? dd[1][p] : dd[2][p]));
     }
     div = NSUB == 1 ? fdiv[0] : (j == 0 ? fdiv[0] : (j == 1 ? fdiv[1] : fdiv[2]));
-    uint32_t t = fresh(px[k]);
+    uint32_t t = texel<ROT>(px, k, sh);
     int proj = 0;
 #pragma unroll
     for (int p = 0; p < CH; p++) proj += (chi(t, p) - ea[p]) * de[p];
@@ -380,9 +386,9 @@ BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pat
 // texels for fixed indices.  Sums are integers (<= 16*15*255 = 61,200);
 // cxx, cyy, cxy and the determinant are exact small integers (det <=
 // 3600^2 < 2^24); the solution keeps the reference's f32 rounding order.
-template <int CH, int BITS>
+template <int CH, int BITS, int ROT = 0>
 BCN_HD void opt_endpoints(const uint32_t px[16], const uint32_t qblock[2], uint32_t mask,
-                          float e0[4], float e1[4]) {
+                          float e0[4], float e1[4], uint32_t sh = 0) {
   constexpr int L1 = (1 << BITS) - 1;
   int sum_q = 0, sum_qq = 0, cnt = 0, sum[4] = {0, 0, 0, 0}, atb1[4] = {0, 0, 0, 0};
 #pragma unroll
@@ -394,7 +400,7 @@ BCN_HD void opt_endpoints(const uint32_t px[16], const uint32_t qblock[2], uint3
     sum_q += qf;
     sum_qq += qf * qf;
     cnt += f;
-    uint32_t t = fresh(px[k]);
+    uint32_t t = texel<ROT>(px, k, sh);
 #pragma unroll
     for (int p = 0; p < CH; p++) {
       int v = f ? chi(t, p) : 0;
@@ -429,9 +435,14 @@ struct ChannelFit {
   uint32_t qblock[2];
 };
 
+// Channel byte at `sh` of texel k: the scalar channel of a mode-4/5 candidate.
+BCN_HD int chan(const uint32_t px[16], int k, uint32_t sh) {
+  return (int)((fresh(px[k]) >> sh) & 0xFFu);
+}
+
 // channel_opt_quant (bc7.rs:278-320).
 template <int BITS>
-BCN_HD int channel_quant(const int cv[16], int e0, int e1, uint32_t qblock[2]) {
+BCN_HD int channel_quant(const uint32_t px[16], uint32_t sh, int e0, int e1, uint32_t qblock[2]) {
   constexpr int L = 1 << BITS;
   float den = (float)(e1 - e0) + 0.001f;
   qblock[0] = 0;
@@ -439,11 +450,12 @@ BCN_HD int channel_quant(const int cv[16], int e0, int e1, uint32_t qblock[2]) {
   int total = 0;
 #pragma unroll
   for (int k = 0; k < 16; k++) {
-    float proj = (float)(cv[k] - e0) / den;
+    const int cvk = chan(px, k, sh);
+    float proj = (float)(cvk - e0) / den;
     int q1 = qclamp(proj * (float)L + 0.5f, 1.0f, (float)(L - 1));
     int w0 = weight<BITS>(q1 - 1), w1 = weight<BITS>(q1);
-    int t0 = (((64 - w0) * e0 + w0 * e1 + 32) >> 6) - cv[k];
-    int t1 = (((64 - w1) * e0 + w1 * e1 + 32) >> 6) - cv[k];
+    int t0 = (((64 - w0) * e0 + w0 * e1 + 32) >> 6) - cvk;
+    int t1 = (((64 - w1) * e0 + w1 * e1 + 32) >> 6) - cvk;
     int err0 = t0 * t0, err1 = t1 * t1;
     bool lo = err0 < err1;
     total += lo ? err0 : err1;
@@ -462,18 +474,19 @@ BCN_HD int channel_qd(float e, int& q) {
 
 // opt_channel (bc7.rs:239-266) with channel_opt_endpoints (bc7.rs:322-369).
 template <int BITS, int EPBITS>
-BCN_HD ChannelFit opt_channel(const int cv[16], uint32_t refine) {
+BCN_HD ChannelFit opt_channel(const uint32_t px[16], uint32_t sh, uint32_t refine) {
   constexpr int L1 = (1 << BITS) - 1;
   int mn = 255, mx = 0;
 #pragma unroll
   for (int k = 0; k < 16; k++) {
-    mn = cv[k] < mn ? cv[k] : mn;
-    mx = cv[k] > mx ? cv[k] : mx;
+    int v = chan(px, k, sh);
+    mn = v < mn ? v : mn;
+    mx = v > mx ? v : mx;
   }
   ChannelFit r;
   int e0 = channel_qd<EPBITS>((float)mn, r.q0);
   int e1 = channel_qd<EPBITS>((float)mx, r.q1);
-  r.err = channel_quant<BITS>(cv, e0, e1, r.qblock);
+  r.err = channel_quant<BITS>(px, sh, e0, e1, r.qblock);
 #pragma unroll 1
   for (uint32_t it = 0; it < refine; it++) {
     // Sums <= 16*15*255; cxx, cyy, cxy, det exact as in opt_endpoints.
@@ -483,8 +496,9 @@ BCN_HD ChannelFit opt_channel(const int cv[16], uint32_t refine) {
       int q = (int)((r.qblock[k >> 3] >> (4 * (k & 7))) & 15u);
       sum_q += q;
       sum_qq += q * q;
-      sum += cv[k];
-      atb1 += (L1 - q) * cv[k];
+      int v = chan(px, k, sh);
+      sum += v;
+      atb1 += (L1 - q) * v;
     }
     int cxx = 16 * L1 * L1 - 2 * L1 * sum_q + sum_qq;
     int cyy = sum_qq;
@@ -501,7 +515,7 @@ BCN_HD ChannelFit opt_channel(const int cv[16], uint32_t refine) {
     x1 = det == 0 ? mean : x1;
     e0 = channel_qd<EPBITS>(x0, r.q0);
     e1 = channel_qd<EPBITS>(x1, r.q1);
-    r.err = channel_quant<BITS>(cv, e0, e1, r.qblock);
+    r.err = channel_quant<BITS>(px, sh, e0, e1, r.qblock);
   }
   return r;
 }
@@ -515,6 +529,10 @@ struct PartFit {
   uint32_t qblock[2];
 };
 
+BCN_HD bool better(const Bc7Best& b, int err, int order) {
+  return err < b.err || (err == b.err && order < b.order);
+}
+
 BCN_HD uint32_t pack4(const int q[4], int ch) {
   uint32_t w = 0;
 #pragma unroll
@@ -523,24 +541,35 @@ BCN_HD uint32_t pack4(const int q[4], int ch) {
   return w;
 }
 
-// bc7_enc_mode01237_part_fast (bc7.rs:517-538) for partition `part_id`
-// (already offset by 64 for 3-subset modes).
+// Quantise + index a partition from its subsets' PCA endpoints e[j][0|1]
+// (the tail of bc7_enc_mode01237_part_fast, bc7.rs:535-537).
 template <int MODE>
-BCN_HD void fit_partition(const uint32_t px[16], int part_id, PartFit& f) {
+BCN_HD void fit_from_segments(const uint32_t px[16], int part_id, const float e[3][2][4],
+                              PartFit& f) {
   constexpr int PAIRS = (MODE == 0 || MODE == 2) ? 3 : 2;
   constexpr int BITS = (MODE == 0 || MODE == 1) ? 3 : 2;
   constexpr int CH = MODE == 7 ? 4 : 3;
   int d[3][2][4];
 #pragma unroll
   for (int j = 0; j < PAIRS; j++) {
-    float e0[4], e1[4];
     int q0[4], q1[4];
-    bc7_segment<CH>(px, part_mask(part_id, j), e0, e1);
-    quant_pair<MODE, CH>(e0, e1, q0, q1, d[j][0], d[j][1]);
+    quant_pair<MODE, CH>(e[j][0], e[j][1], q0, q1, d[j][0], d[j][1]);
     f.q[j][0] = pack4(q0, CH);
     f.q[j][1] = pack4(q1, CH);
   }
   f.err = block_quant<CH, BITS, PAIRS>(px, d, kPart.subset[part_id], f.qblock);
+}
+
+// bc7_enc_mode01237_part_fast (bc7.rs:517-538) for partition `part_id`
+// (already offset by 64 for 3-subset modes).
+template <int MODE>
+BCN_HD void fit_partition(const uint32_t px[16], int part_id, PartFit& f) {
+  constexpr int PAIRS = (MODE == 0 || MODE == 2) ? 3 : 2;
+  constexpr int CH = MODE == 7 ? 4 : 3;
+  float e[3][2][4];
+#pragma unroll
+  for (int j = 0; j < PAIRS; j++) bc7_segment<CH>(px, part_mask(part_id, j), e[j][0], e[j][1]);
+  fit_from_segments<MODE>(px, part_id, e, f);
 }
 
 // The refinement step of bc7_enc_mode01237 (bc7.rs:571-600).
@@ -564,9 +593,10 @@ BCN_HD void refine_partition(const uint32_t px[16], int part_id, const uint32_t 
 }
 
 template <int MODE>
-BCN_HD void store_partfit(Bc7Best& b, const PartFit& f, int err, int part_id) {
+BCN_HD void store_partfit(Bc7Best& b, const PartFit& f, int err, int part_id, int order) {
   constexpr int PAIRS = (MODE == 0 || MODE == 2) ? 3 : 2;
   b.err = err;
+  b.order = order;
   b.mode = MODE;
   b.part = (uint32_t)part_id;
 #pragma unroll
@@ -577,19 +607,31 @@ BCN_HD void store_partfit(Bc7Best& b, const PartFit& f, int err, int part_id) {
   b.qblock[1] = f.qblock[1];
 }
 
-// bc7_enc_mode01237 (bc7.rs:540-610) over `count` candidates taken from
-// `list` (a partition id in the low 6 bits of each entry, read with stride).
+// Refinement and final comparison of bc7_enc_mode01237 (bc7.rs:571-609) for
+// the best candidate bf of partition part_id.
 template <int MODE>
-BCN_HD void enc_mode01237(const uint32_t px[16], const Bc7Params& s, const int* list, int stride,
-                          int count, bool identity_list, int opaque_err, Bc7Best& best) {
-  constexpr int PAIRS = (MODE == 0 || MODE == 2) ? 3 : 2;
-  if (count == 0) return;
-  PartFit bf;
-  int best_part = -1;
-  bf.err = 0x7FFFFFFF;
+BCN_HD void finish_mode01237(const uint32_t px[16], const Bc7Params& s, PartFit bf, int part_id,
+                             int opaque_err, int order, Bc7Best& best) {
 #pragma unroll 1
-  for (int i = 0; i < count; i++) {
-    int part = identity_list ? i : (list[i * stride] & 63);
+  for (uint32_t it = 0; it < s.refine_iterations[MODE]; it++) {
+    PartFit f;
+    refine_partition<MODE>(px, part_id, bf.qblock, f);
+    if (f.err < bf.err) bf = f;
+  }
+  int err = MODE != 7 ? bf.err + opaque_err : bf.err;
+  if (better(best, err, order)) store_partfit<MODE>(best, bf, err, part_id, order);
+}
+
+// bc7_enc_mode02's calls (bc7.rs:612-620): every partition 0..count-1, in order.
+template <int MODE>
+BCN_HD void enc_mode02(const uint32_t px[16], const Bc7Params& s, int count, int opaque_err,
+                       int order, Bc7Best& best) {
+  constexpr int PAIRS = (MODE == 0 || MODE == 2) ? 3 : 2;
+  PartFit bf;
+  bf.err = 0x7FFFFFFF;
+  int best_part = 0;
+#pragma unroll 1
+  for (int part = 0; part < count; part++) {
     int part_id = PAIRS == 3 ? part + 64 : part;
     PartFit f;
     fit_partition<MODE>(px, part_id, f);
@@ -598,19 +640,15 @@ BCN_HD void enc_mode01237(const uint32_t px[16], const Bc7Params& s, const int* 
       best_part = part_id;
     }
   }
-#pragma unroll 1
-  for (uint32_t it = 0; it < s.refine_iterations[MODE]; it++) {
-    PartFit f;
-    refine_partition<MODE>(px, best_part, bf.qblock, f);
-    if (f.err < bf.err) bf = f;
-  }
-  int err = MODE != 7 ? bf.err + opaque_err : bf.err;
-  if (err < best.err) store_partfit<MODE>(best, bf, err, best_part);
+  finish_mode01237<MODE>(px, s, bf, best_part, opaque_err, order, best);
 }
 
-// Write the 64 partition keys part + bound*64 (bc7.rs:629-638, 824-843).
+// PCA split bounds of the 64 two-subset partitions (bc7.rs:629-638,
+// 824-843): keys[part * stride] = (int)bound.  For 8-bit texels a bound is at
+// most sqrt(2 * 4 * 255^2 * 16/4 / 65536) * 256 < 1450, so it fits 16 bits
+// and the reference's key part + bound * 64 orders like (bound, part).
 template <int CH>
-BCN_HD void partition_keys(const uint32_t px[16], int* keys, int stride) {
+BCN_HD void partition_bounds(const uint32_t px[16], uint16_t* keys, int stride) {
   Moments full = moments<CH>(px, 0xFFFFu);
 #pragma unroll 1
   for (int part = 0; part < 64; part++) {
@@ -619,23 +657,23 @@ BCN_HD void partition_keys(const uint32_t px[16], int* keys, int stride) {
     float bound = 0.0f;
 #pragma unroll
     for (int side = 0; side < 2; side++) {
-      int s[4], q[10], n;
+      int sm[4], q[10], n;
       if (side == 0) {
         n = m.n;
 #pragma unroll
-        for (int i = 0; i < 4; i++) s[i] = m.s[i];
+        for (int i = 0; i < 4; i++) sm[i] = m.s[i];
 #pragma unroll
         for (int i = 0; i < 10; i++) q[i] = m.q[i];
       } else {
         // full_stats - stats: exact integer differences.
         n = full.n - m.n;
 #pragma unroll
-        for (int i = 0; i < 4; i++) s[i] = full.s[i] - m.s[i];
+        for (int i = 0; i < 4; i++) sm[i] = full.s[i] - m.s[i];
 #pragma unroll
         for (int i = 0; i < 10; i++) q[i] = full.q[i] - m.q[i];
       }
       float c[10];
-      covar_from_moments<CH>(c, s, q, n);
+      covar_from_moments<CH>(c, sm, q, n);
       // get_pca_bound (common.rs:208-245).
       const float inv_var = 1.0f / (256.0f * 256.0f);
       const float eps = 0.001f * 0.001f;
@@ -660,77 +698,132 @@ BCN_HD void partition_keys(const uint32_t px[16], int* keys, int stride) {
       float sq_sum = sq(a0) + sq(a1) + sq(a2);
       if (CH == 4) sq_sum = sq_sum + sq(a3);
       float lambda = sqrtf(sq_sum);
-      float b = c[0] + c[4] + c[7];
-      if (CH == 4) b = b + c[9];
-      b = b - lambda;
-      b = fmaxf(b, 0.0f);
+      float bb = c[0] + c[4] + c[7];
+      if (CH == 4) bb = bb + c[9];
+      bb = bb - lambda;
+      bb = fmaxf(bb, 0.0f);
       // `bound += b`: the first add is 0.0 + b == b for b >= +0.0.
-      bound = side == 0 ? b : bound + b;
+      bound = side == 0 ? bb : bound + bb;
     }
     float bound12 = sqrtf(bound) * 256.0f;
-    int ibound = (int)bound12;  // finite, non-negative, far below 2^31
-    keys[part * stride] = part + ibound * 64;
+    keys[part * stride] = (uint16_t)(int)bound12;  // finite, 0 <= bound12 < 1450
   }
 }
 
-// In place: the first `count` entries become the `count` smallest keys in
-// ascending order (keys are unique).
-BCN_HD void select_smallest(int* keys, int stride, int count) {
-  int last = -2147483647 - 1;
+// Next partition in ascending key order (partial_sort_list, common.rs:126-140,
+// consumed one entry at a time); marks it taken with 0xFFFF, which sorts after
+// every real key.
+BCN_HD int select_next(uint16_t* keys, int stride) {
+  int best = 0x7FFFFFFF, best_i = 0;
+#pragma unroll 8
+  for (int i = 0; i < 64; i++) {
+    int key = (int)keys[i * stride] * 64 + i;
+    bool take = key < best;
+    best = take ? key : best;
+    best_i = take ? i : best_i;
+  }
+  keys[best_i * stride] = 0xFFFFu;
+  return best_i;
+}
+
+// bc7_enc_mode13 (bc7.rs:622-655).  Mode 1 takes the t1 smallest keys and
+// mode 3 the t3 smallest, from the same ascending list; a partition's two
+// PCA fits (3 channels, its two subset masks) are identical for both modes,
+// so each listed partition is fitted once and quantised for both.
+BCN_HD void enc_mode13(const uint32_t px[16], const Bc7Params& s, uint16_t* keys, int stride,
+                       int opaque_err, Bc7Best& best) {
+  const int t1 = (int)s.fast_skip_threshold_mode1, t3 = (int)s.fast_skip_threshold_mode3;
+  if (t1 == 0 && t3 == 0) return;
+  partition_bounds<3>(px, keys, stride);
+  const int n = t1 > t3 ? t1 : t3;
+  PartFit b1, b3;
+  b1.err = b3.err = 0x7FFFFFFF;
+  int p1 = 0, p3 = 0;
 #pragma unroll 1
-  for (int t = 0; t < count; t++) {
-    int best = 0x7FFFFFFF, best_i = 0;
-#pragma unroll 4
-    for (int i = t; i < 64; i++) {
-      int v = keys[i * stride];
-      bool take = v < best && v > last;
-      best = take ? v : best;
-      best_i = take ? i : best_i;
+  for (int r = 0; r < n; r++) {
+    int part = select_next(keys, stride);
+    float e[3][2][4];
+#pragma unroll
+    for (int j = 0; j < 2; j++) bc7_segment<3>(px, part_mask(part, j), e[j][0], e[j][1]);
+    if (r < t1) {
+      PartFit f;
+      fit_from_segments<1>(px, part, e, f);
+      if (f.err < b1.err) {
+        b1 = f;
+        p1 = part;
+      }
     }
-    // Swap the minimum into slot t; entries before t are all <= last.
-    int tmp = keys[t * stride];
-    keys[t * stride] = best;
-    keys[best_i * stride] = tmp;
-    last = best;
+    if (r < t3) {
+      // Keep the mode-3 fit after the mode-1 fit (see `after`).
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+          for (int p = 0; p < 3; p++) after(e[j][i][p], b1.err);
+      PartFit f;
+      fit_from_segments<3>(px, part, e, f);
+      if (f.err < b3.err) {
+        b3 = f;
+        p3 = part;
+      }
+    }
   }
+  if (t1 != 0) finish_mode01237<1>(px, s, b1, p1, opaque_err, 2, best);
+  if (t3 != 0) finish_mode01237<3>(px, s, b3, p3, opaque_err, 3, best);
 }
 
-// bc7_enc_mode45_candidate (bc7.rs:657-733).
-template <int MODE, int CH, int SWAP>
-BCN_HD void enc_mode45_candidate(const uint32_t px[16], const Bc7Params& s, uint32_t rotation,
-                                 Bc7Best& best) {
+// bc7_enc_mode7 (bc7.rs:819-851).
+template <int CH>
+BCN_HD void enc_mode7(const uint32_t px[16], const Bc7Params& s, uint16_t* keys, int stride,
+                      Bc7Best& best) {
+  const int t7 = (int)s.fast_skip_threshold_mode7;
+  if (t7 == 0) return;
+  partition_bounds<CH>(px, keys, stride);
+  PartFit bf;
+  bf.err = 0x7FFFFFFF;
+  int p7 = 0;
+#pragma unroll 1
+  for (int r = 0; r < t7; r++) {
+    int part = select_next(keys, stride);
+    PartFit f;
+    fit_partition<7>(px, part, f);
+    if (f.err < bf.err) {
+      bf = f;
+      p7 = part;
+    }
+  }
+  finish_mode01237<7>(px, s, bf, p7, 0, 4, best);
+}
+
+// bc7_enc_mode45_candidate (bc7.rs:657-733) for rotation `rotation`: the
+// rotated block is read through texel<ROT> (channel byte sh = 8*rotation),
+// the scalar channel is byte sh of the original texels, and se0/se1 are the
+// rotated block's clamped PCA endpoints (block_segment of the rotated block is
+// the same for mode 4 swap 0/1 and mode 5).
+template <int MODE, int SWAP, int ROT>
+BCN_HD void mode45_candidate(const uint32_t px[16], uint32_t sh, const float se0[4],
+                             const float se1[4], const Bc7Params& s, uint32_t rotation, int order,
+                             Bc7Best& best) {
   constexpr int BITS = SWAP ? 3 : 2;
   constexpr int ABITS = SWAP ? 2 : (MODE == 4 ? 3 : 2);
   constexpr int AEPBITS = MODE == 4 ? 6 : 8;
-  uint32_t cb[16];
-  int cv[16];
-#pragma unroll
-  for (int k = 0; k < 16; k++) {
-    uint32_t v = px[k] & 0x00FFFFFFu;
-    if (rotation < 3) {
-      uint32_t sh = 8u * rotation;
-      uint32_t repl = CH == 4 ? (px[k] >> 24) : 255u;
-      v = (v & ~(0xFFu << sh)) | (repl << sh);
-    }
-    cb[k] = v;
-    cv[k] = chi(px[k], (int)rotation);
-  }
-  float e0[4], e1[4];
-  bc7_segment<3>(cb, 0xFFFFu, e0, e1);
+  float e0[4] = {se0[0], se0[1], se0[2], se0[3]}, e1[4] = {se1[0], se1[1], se1[2], se1[3]};
   int q[2][4], d[3][2][4];
   quant_pair<MODE, 3>(e0, e1, q[0], q[1], d[0][0], d[0][1]);
   uint32_t qblock[2];
-  int err = block_quant<3, BITS, 1>(cb, d, 0u, qblock);
+  int err = block_quant<3, BITS, 1, ROT>(px, d, 0u, qblock, sh);
 #pragma unroll 1
   for (uint32_t it = 0; it < s.refine_iterations[MODE]; it++) {
-    opt_endpoints<3, BITS>(cb, qblock, 0xFFFFu, e0, e1);
+    opt_endpoints<3, BITS, ROT>(px, qblock, 0xFFFFu, e0, e1, sh);
     quant_pair<MODE, 3>(e0, e1, q[0], q[1], d[0][0], d[0][1]);
-    err = block_quant<3, BITS, 1>(cb, d, 0u, qblock);
+    err = block_quant<3, BITS, 1, ROT>(px, d, 0u, qblock, sh);
   }
-  ChannelFit a = opt_channel<ABITS, AEPBITS>(cv, s.refine_iterations_channel);
+  ChannelFit a = opt_channel<ABITS, AEPBITS>(px, sh, s.refine_iterations_channel);
   err += a.err;
-  if (err < best.err) {
+  if (better(best, err, order)) {
     best.err = err;
+    best.order = order;
     best.mode = MODE;
     best.rotation = rotation;
     best.swap = SWAP;
@@ -742,6 +835,31 @@ BCN_HD void enc_mode45_candidate(const uint32_t px[16], const Bc7Params& s, uint
     best.qblock[1] = qblock[1];
     best.aqblock[0] = a.qblock[0];
     best.aqblock[1] = a.qblock[1];
+  }
+}
+
+template <int ROT>
+BCN_HD void mode45_rotation(const uint32_t px[16], const Bc7Params& s, uint32_t rot, Bc7Best& best) {
+  const uint32_t sh = 8u * rot;
+  float e0[4], e1[4];
+  bc7_segment<3, ROT>(px, 0xFFFFu, e0, e1, sh);
+  const int r = (int)(rot - s.mode45_channel0);
+  mode45_candidate<4, 0, ROT>(px, sh, e0, e1, s, rot, 10 + 2 * r, best);
+  mode45_candidate<4, 1, ROT>(px, sh, e0, e1, s, rot, 11 + 2 * r, best);
+  mode45_candidate<5, 0, ROT>(px, sh, e0, e1, s, rot, 20 + r, best);
+}
+
+// bc7_enc_mode45 (bc7.rs:735-760).  Per rotation: the rotated block's PCA
+// fit once, then mode 4 swap 0, mode 4 swap 1 and mode 5.  The reference
+// visits all mode-4 candidates before the mode-5 ones; the `order` values
+// keep its tie-breaking (mode 4: 10 + 2*r + swap, mode 5: 20 + r).
+template <int CH>
+BCN_HD void enc_mode45(const uint32_t px[16], const Bc7Params& s, Bc7Best& best) {
+#pragma unroll 1
+  for (uint32_t rot = s.mode45_channel0; rot < s.channels; rot++) {
+    if (rot == 3) mode45_rotation<0>(px, s, rot, best);  // no rotation (channels == 4)
+    else if (CH == 4) mode45_rotation<2>(px, s, rot, best);
+    else mode45_rotation<1>(px, s, rot, best);
   }
 }
 
@@ -770,8 +888,9 @@ BCN_HD void enc_mode6(const uint32_t px[16], const Bc7Params& s, Bc7Best& best) 
     quant_pair<6, CH>(e0, e1, q[0], q[1], d[0][0], d[0][1]);
     err = block_quant<CH, 4, 1>(px, d, 0u, qblock);
   }
-  if (err < best.err) {
+  if (better(best, err, 30)) {
     best.err = err;
+    best.order = 30;
     best.mode = 6;
     best.qep[0] = (uint32_t)q[0][0] | ((uint32_t)q[0][1] << 8) | ((uint32_t)q[0][2] << 16) |
                   ((uint32_t)q[0][3] << 24);
@@ -796,20 +915,24 @@ BCN_HD void write_out(const BitWriter& w, uint32_t out[4]) {
   out[3] = (uint32_t)(w.hi >> 32);
 }
 
+template <int MODE>
 BCN_HD void code_mode01237(const Bc7Best& b, uint32_t out[4]) {
-  const int mode = (int)b.mode;
+  constexpr int bits = (MODE == 0 || MODE == 1) ? 3 : 2;
+  constexpr int pairs = (MODE == 0 || MODE == 2) ? 3 : 2;
+  constexpr int channels = MODE == 7 ? 4 : 3;
+  constexpr uint32_t levels = 1u << bits;
   const int part = (int)b.part;
-  const int bits = (mode == 0 || mode == 1) ? 3 : 2;
-  const int pairs = (mode == 0 || mode == 2) ? 3 : 2;
-  const int channels = mode == 7 ? 4 : 3;
-  const uint32_t levels = 1u << bits;
   uint32_t qep[6];
+#pragma unroll
   for (int e = 0; e < 6; e++) qep[e] = b.qep[e];
   // Endpoint swap so that each subset's anchor index has MSB 0
   // (bc7_code_apply_swap_mode01237, common.rs:583-613).
+  const int a1 = kPart.anchor1[part];
+  const int a2 = pairs > 2 ? kPart.anchor2[part] : -1;
   uint32_t flips = 0;
+#pragma unroll
   for (int j = 0; j < pairs; j++) {
-    int k0 = j == 0 ? 0 : (j == 1 ? kPart.anchor1[part] : kPart.anchor2[part]);
+    int k0 = j == 0 ? 0 : (j == 1 ? a1 : a2);
     if (qidx(b.qblock, k0) >= levels / 2) {
       uint32_t t = qep[2 * j];
       qep[2 * j] = qep[2 * j + 1];
@@ -818,24 +941,27 @@ BCN_HD void code_mode01237(const Bc7Best& b, uint32_t out[4]) {
     }
   }
   BitWriter w;
-  w.put((uint32_t)mode + 1, 1u << mode);
-  if (mode == 0) w.put(4, (uint32_t)(part & 15));
+  w.put((uint32_t)MODE + 1, 1u << MODE);
+  if (MODE == 0) w.put(4, (uint32_t)(part & 15));
   else w.put(6, (uint32_t)(part & 63));
+#pragma unroll
   for (int p = 0; p < channels; p++)
+#pragma unroll
     for (int e = 0; e < pairs * 2; e++) {
       uint32_t v = qbyte(qep, e, p);
-      if (mode == 0) w.put(4, v >> 1);
-      else if (mode == 1) w.put(6, v >> 1);
-      else if (mode == 2) w.put(5, v);
-      else if (mode == 3) w.put(7, v >> 1);
+      if (MODE == 0) w.put(4, v >> 1);
+      else if (MODE == 1) w.put(6, v >> 1);
+      else if (MODE == 2) w.put(5, v);
+      else if (MODE == 3) w.put(7, v >> 1);
       else w.put(5, v >> 1);
     }
-  if (mode == 1)
+  if (MODE == 1)
+#pragma unroll
     for (int j = 0; j < 2; j++) w.put(1, qbyte(qep, 2 * j, 0) & 1u);
-  if (mode == 0 || mode == 3 || mode == 7)
+  if (MODE == 0 || MODE == 3 || MODE == 7)
+#pragma unroll
     for (int e = 0; e < pairs * 2; e++) w.put(1, qbyte(qep, e, 0) & 1u);
-  int a1 = pairs > 1 ? kPart.anchor1[part] : -1;
-  int a2 = pairs > 2 ? kPart.anchor2[part] : -1;
+#pragma unroll
   for (int k = 0; k < 16; k++) {
     uint32_t q = qidx(b.qblock, k);
     if ((flips >> k) & 1u) q = levels - 1 - q;
@@ -914,12 +1040,13 @@ BCN_HD void code_mode6(const Bc7Best& b, uint32_t out[4]) {
 }
 
 // compress_block_bc7_core (bc7.rs:853-867) for one block.  `keys` is a
-// 64-entry scratch row read and written with `stride`.
+// 64-entry scratch row (16-bit) read and written with `stride`.
 template <int CH>
-BCN_HD void encode_bc7_block(const uint32_t px[16], const Bc7Params& s, int* keys, int stride,
+BCN_HD void encode_bc7_block(const uint32_t px[16], const Bc7Params& s, uint16_t* keys, int stride,
                              uint32_t out[4]) {
   Bc7Best best;
   best.err = 0x7FFFFFFF;  // stands for the reference's f32::INFINITY
+  best.order = 0x7FFFFFFF;
   best.mode = 8;
   best.part = best.rotation = best.swap = 0;
   for (int e = 0; e < 6; e++) best.qep[e] = 0;
@@ -934,47 +1061,44 @@ BCN_HD void encode_bc7_block(const uint32_t px[16], const Bc7Params& s, int* key
       opaque_err += t * t;
     }
 
+  // Each mode group starts from laundered texels (see `fresh`), so the
+  // compiler cannot carry values derived from them from one group into the
+  // next and keep them live across the whole search.
+  uint32_t t[16];
+#pragma unroll
+  for (int k = 0; k < 16; k++) t[k] = fresh(px[k]);
   if (s.mode_selection[0] != 0) {
-    enc_mode01237<0>(px, s, keys, stride, 16, true, opaque_err, best);
-    if (s.skip_mode2 == 0) enc_mode01237<2>(px, s, keys, stride, 64, true, opaque_err, best);
+    enc_mode02<0>(t, s, 16, opaque_err, 0, best);
+    if (s.skip_mode2 == 0) enc_mode02<2>(t, s, 64, opaque_err, 1, best);
   }
+#pragma unroll
+  for (int k = 0; k < 16; k++) t[k] = fresh(t[k]);
   if (s.mode_selection[1] != 0) {
-    uint32_t t1 = s.fast_skip_threshold_mode1, t3 = s.fast_skip_threshold_mode3;
-    if (t1 != 0 || t3 != 0) {
-      partition_keys<3>(px, keys, stride);
-      select_smallest(keys, stride, (int)(t1 > t3 ? t1 : t3));
-      enc_mode01237<1>(px, s, keys, stride, (int)t1, false, opaque_err, best);
-      enc_mode01237<3>(px, s, keys, stride, (int)t3, false, opaque_err, best);
-    }
-    uint32_t t7 = s.fast_skip_threshold_mode7;
-    if (t7 != 0) {
-      partition_keys<CH>(px, keys, stride);
-      select_smallest(keys, stride, (int)t7);
-      enc_mode01237<7>(px, s, keys, stride, (int)t7, false, opaque_err, best);
-    }
+    enc_mode13(t, s, keys, stride, opaque_err, best);
+#pragma unroll
+    for (int k = 0; k < 16; k++) t[k] = fresh(t[k]);
+    enc_mode7<CH>(t, s, keys, stride, best);
   }
-  if (s.mode_selection[2] != 0) {
-#pragma unroll 1
-    for (uint32_t p = s.mode45_channel0; p < s.channels; p++) {
-      enc_mode45_candidate<4, CH, 0>(px, s, p, best);
-      enc_mode45_candidate<4, CH, 1>(px, s, p, best);
-    }
-#pragma unroll 1
-    for (uint32_t p = s.mode45_channel0; p < s.channels; p++)
-      enc_mode45_candidate<5, CH, 0>(px, s, p, best);
-  }
-  if (s.mode_selection[3] != 0) enc_mode6<CH>(px, s, best);
+#pragma unroll
+  for (int k = 0; k < 16; k++) t[k] = fresh(t[k]);
+  if (s.mode_selection[2] != 0) enc_mode45<CH>(t, s, best);
+#pragma unroll
+  for (int k = 0; k < 16; k++) t[k] = fresh(t[k]);
+  if (s.mode_selection[3] != 0) enc_mode6<CH>(t, s, best);
 
-  if (best.mode == 8) {
-    // No candidate beat the initial error: the reference leaves its all-zero
-    // data words in place.
-    out[0] = out[1] = out[2] = out[3] = 0;
-  } else if (best.mode == 4 || best.mode == 5) {
-    code_mode45(best, out);
-  } else if (best.mode == 6) {
-    code_mode6(best, out);
-  } else {
-    code_mode01237(best, out);
+  switch (best.mode) {
+    case 0: code_mode01237<0>(best, out); break;
+    case 1: code_mode01237<1>(best, out); break;
+    case 2: code_mode01237<2>(best, out); break;
+    case 3: code_mode01237<3>(best, out); break;
+    case 7: code_mode01237<7>(best, out); break;
+    case 4:
+    case 5: code_mode45(best, out); break;
+    case 6: code_mode6(best, out); break;
+    default:
+      // No candidate beat the initial error: the reference leaves its
+      // all-zero data words in place.
+      out[0] = out[1] = out[2] = out[3] = 0;
   }
 }
 

@@ -6,7 +6,7 @@
 // (neighbouring lanes read neighbouring 16-byte chunks, so a wave reads four
 // contiguous 1 KiB row segments), runs the block search in bc7_block.hpp
 // and writes 16 bytes.  The 64 mode-1/3/7 partition keys of each lane live in
-// LDS, interleaved by lane so the key scans are bank-conflict free.
+// LDS as 16-bit bounds (8 KiB per 64-lane workgroup), interleaved by lane.
 #include "bc7_block.hpp"
 #include "bcn_internal.hpp"
 
@@ -14,13 +14,22 @@ namespace bcn {
 
 constexpr int kBc7Wave = 64;
 
+// Occupancy target of the BC7 kernel in waves per SIMD.  Unconstrained, the
+// block search takes about 240 VGPRs (2 waves); at 3 waves it fits 168 VGPRs
+// with 16 bytes per lane spilled.  On 4096x4096 opaque_basic, 2/3/4 waves ran
+// 7.04/6.91/6.95 ms per launch.  The 8 KiB of LDS per workgroup allows 5.
+#ifndef BCN_BC7_WAVES
+#define BCN_BC7_WAVES 3
+#endif
+
 template <int CH, bool VEC>
-__global__ __launch_bounds__(kBc7Wave) void bc7_encode_kernel(const uint8_t* __restrict__ src,
+__global__ __launch_bounds__(kBc7Wave)
+__attribute__((amdgpu_waves_per_eu(BCN_BC7_WAVES, BCN_BC7_WAVES))) void bc7_encode_kernel(const uint8_t* __restrict__ src,
                                                               uint32_t pitch, uint32_t bw,
                                                               uint32_t nblocks,
                                                               uint32_t* __restrict__ dst,
                                                               Bc7Params s) {
-  __shared__ int keys[64 * kBc7Wave];
+  __shared__ uint16_t keys[64 * kBc7Wave];
   uint32_t idx = blockIdx.x * kBc7Wave + threadIdx.x;
   if (idx >= nblocks) return;
   uint32_t yy = idx / bw, xx = idx - yy * bw;

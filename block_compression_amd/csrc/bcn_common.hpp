@@ -55,6 +55,29 @@ BCN_HD uint32_t fresh(uint32_t x) {
   return x;
 }
 
+// Makes x depend on `dep` without changing it: code reading x cannot be
+// scheduled before `dep` is computed.  Used to keep two independent fits in
+// sequence instead of interleaved (interleaving doubles the live registers).
+BCN_HD void after(float& x, int dep) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm volatile("" : "+v"(x) : "v"(dep));
+#else
+  asm volatile("" : "+r"(x) : "r"(dep));
+#endif
+}
+
+// Texel k of a block as the mode-4/5 candidates see it (bc7.rs:680-696):
+// ROT 0 = as stored; ROT 1 = channel byte at `sh` forced to 255 (3-channel
+// rotation); ROT 2 = channel byte at `sh` replaced by alpha (4-channel).
+// Consumers read only bytes 0..2.
+template <int ROT>
+BCN_HD uint32_t texel(const uint32_t px[16], int k, uint32_t sh) {
+  uint32_t t = fresh(px[k]);
+  if (ROT == 1) return t | (0xFFu << sh);
+  if (ROT == 2) return (t & ~(0xFFu << sh)) | ((t >> 24) << sh);
+  return t;
+}
+
 // Byte p of a packed RGBA8 texel (a single v_cvt_f32_ubyteN / v_bfe on gfx950).
 BCN_HD float chf(uint32_t px, int p) { return (float)((px >> (8 * p)) & 0xFFu); }
 BCN_HD int chi(uint32_t px, int p) { return (int)((px >> (8 * p)) & 0xFFu); }
