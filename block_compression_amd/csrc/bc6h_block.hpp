@@ -307,8 +307,11 @@ BCN_HD float fblock_quant(const Bc6hBlock& b, const int d[2][2][3], uint32_t pat
     float err0 = 0.0f, err1 = 0.0f;
 #pragma unroll
     for (int p = 0; p < 3; p++) {
-      float v0 = (float)(((64 - w0) * ia[p] + w0 * ib[p] + 32) >> 6);
-      float v1 = (float)(((64 - w1) * ia[p] + w1 * ib[p] + 32) >> 6);
+      // ((64 - w) * a + w * b + 32) >> 6 == a + ((w * (b - a) + 32) >> 6):
+      // 64 * a is a multiple of 64 and >> is floor division.  Endpoints are
+      // 16-bit, so w * (b - a) is below 2^22 in magnitude.
+      float v0 = (float)(ia[p] + ((mul24(w0, ib[p] - ia[p]) + 32) >> 6));
+      float v1 = (float)(ia[p] + ((mul24(w1, ib[p] - ia[p]) + 32) >> 6));
       err0 = err0 + sq(v0 - b.blk[p * 16 + k]);
       err1 = err1 + sq(v1 - b.blk[p * 16 + k]);
     }

@@ -339,7 +339,7 @@ BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pat
 #pragma unroll
     for (int p = 0; p < CH; p++) {
       dd[j][p] = d[j][1][p] - d[j][0][p];
-      div += dd[j][p] * dd[j][p];
+      div += mul24(dd[j][p], dd[j][p]);
     }
     fdiv[j] = (float)div;
   }
@@ -360,19 +360,21 @@ BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pat
     uint32_t t = texel<ROT>(px, k, sh);
     int proj = 0;
 #pragma unroll
-    for (int p = 0; p < CH; p++) proj += (chi(t, p) - ea[p]) * de[p];
+    for (int p = 0; p < CH; p++) proj += mul24(chi(t, p) - ea[p], de[p]);
     float fp = (float)proj / div;
     int q1 = qclamp(fp * (float)L + 0.5f, 1.0f, (float)(L - 1));
     int w0 = weight<BITS>(q1 - 1), w1 = weight<BITS>(q1);
     int err0 = 0, err1 = 0;
 #pragma unroll
     for (int p = 0; p < CH; p++) {
-      int eb = ea[p] + de[p];
-      int v = chi(t, p);
-      int t0 = (((64 - w0) * ea[p] + w0 * eb + 32) >> 6) - v;
-      int t1 = (((64 - w1) * ea[p] + w1 * eb + 32) >> 6) - v;
-      err0 += t0 * t0;
-      err1 += t1 * t1;
+      // The reference's ((64 - w) * a + w * b + 32) >> 6 with b = a + de: the
+      // 64 * a term is a multiple of 64 and >> is floor division, so it equals
+      // a + ((w * de + 32) >> 6).
+      int base = ea[p] - chi(t, p);
+      int t0 = base + ((mul24(w0, de[p]) + 32) >> 6);
+      int t1 = base + ((mul24(w1, de[p]) + 32) >> 6);
+      err0 += mul24(t0, t0);
+      err1 += mul24(t1, t1);
     }
     bool lo = err0 < err1;
     int q = lo ? q1 - 1 : q1;
@@ -405,19 +407,19 @@ BCN_HD void opt_endpoints(const uint32_t px[16], const uint32_t qblock[2], uint3
     for (int p = 0; p < CH; p++) {
       int v = f ? chi(t, p) : 0;
       sum[p] += v;
-      atb1[p] += xf * v;
+      atb1[p] += mul24(xf, v);
     }
   }
-  int cxx = cnt * L1 * L1 - 2 * L1 * sum_q + sum_qq;
+  int cxx = mul24i(cnt, L1 * L1) - mul24i(2 * L1, sum_q) + sum_qq;
   int cyy = sum_qq;
-  int cxy = L1 * sum_q - sum_qq;
-  int det = cxx * cyy - cxy * cxy;
+  int cxy = mul24i(L1, sum_q) - sum_qq;
+  int det = mul24i(cxx, cyy) - mul24i(cxy, cxy);
   float scale = (float)L1 / (float)det;
   float fcxx = (float)cxx, fcyy = (float)cyy, fcxy = (float)cxy;
 #pragma unroll
   for (int p = 0; p < CH; p++) {
     float a1 = (float)atb1[p];
-    float a2 = (float)(L1 * sum[p] - atb1[p]);
+    float a2 = (float)(mul24(L1, sum[p]) - atb1[p]);
     float x0 = (a1 * fcyy - a2 * fcxy) * scale;
     float x1 = (a2 * fcxx - a1 * fcxy) * scale;
     // |det| < 0.001 with an integer det: flatten to the mean.
@@ -454,9 +456,10 @@ BCN_HD int channel_quant(const uint32_t px[16], uint32_t sh, int e0, int e1, uin
     float proj = (float)(cvk - e0) / den;
     int q1 = qclamp(proj * (float)L + 0.5f, 1.0f, (float)(L - 1));
     int w0 = weight<BITS>(q1 - 1), w1 = weight<BITS>(q1);
-    int t0 = (((64 - w0) * e0 + w0 * e1 + 32) >> 6) - cvk;
-    int t1 = (((64 - w1) * e0 + w1 * e1 + 32) >> 6) - cvk;
-    int err0 = t0 * t0, err1 = t1 * t1;
+    // Interpolation as in block_quant: a + ((w * (b - a) + 32) >> 6).
+    int t0 = e0 - cvk + ((mul24(w0, e1 - e0) + 32) >> 6);
+    int t1 = e0 - cvk + ((mul24(w1, e1 - e0) + 32) >> 6);
+    int err0 = mul24(t0, t0), err1 = mul24(t1, t1);
     bool lo = err0 < err1;
     total += lo ? err0 : err1;
     qblock[k >> 3] |= (uint32_t)(lo ? q1 - 1 : q1) << (4 * (k & 7));
@@ -498,14 +501,14 @@ BCN_HD ChannelFit opt_channel(const uint32_t px[16], uint32_t sh, uint32_t refin
       sum_qq += q * q;
       int v = chan(px, k, sh);
       sum += v;
-      atb1 += (L1 - q) * v;
+      atb1 += mul24(L1 - q, v);
     }
-    int cxx = 16 * L1 * L1 - 2 * L1 * sum_q + sum_qq;
+    int cxx = 16 * L1 * L1 - mul24i(2 * L1, sum_q) + sum_qq;
     int cyy = sum_qq;
-    int cxy = L1 * sum_q - sum_qq;
-    int det = cxx * cyy - cxy * cxy;
+    int cxy = mul24i(L1, sum_q) - sum_qq;
+    int det = mul24i(cxx, cyy) - mul24i(cxy, cxy);
     float scale = (float)L1 / (float)det;
-    float a1 = (float)atb1, a2 = (float)(L1 * sum - atb1);
+    float a1 = (float)atb1, a2 = (float)(mul24(L1, sum) - atb1);
     float x0 = (a1 * (float)cyy - a2 * (float)cxy) * scale;
     float x1 = (a2 * (float)cxx - a1 * (float)cxy) * scale;
     x0 = rclamp(x0, 0.0f, 255.0f);

@@ -43,6 +43,30 @@ BCN_HD float rclamp(float x, float lo, float hi) {
 
 BCN_HD int iclamp(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
 
+// a * b for operands in [-2^23, 2^23) whose product fits in int32.  gfx950
+// multiplies 24-bit operands at full rate (v_mul_i32_i24, v_mad_i32_i24) and
+// 32-bit ones at quarter rate (v_mul_lo_u32).  The compiler picks the fast
+// form only where instruction selection can prove the range inside one basic
+// block, which it mostly cannot here, so the intrinsic is called directly
+// (clang has no builtin for it; the asm label names the LLVM intrinsic).
+#if defined(__HIP_DEVICE_COMPILE__)
+extern "C" __device__ int bcn_llvm_mul_i24(int, int) __asm("llvm.amdgcn.mul.i24");
+#endif
+BCN_HD int mul24(int a, int b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __mul24(a, b);
+#else
+  return a * b;
+#endif
+}
+BCN_HD int mul24i(int a, int b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return bcn_llvm_mul_i24(a, b);
+#else
+  return a * b;
+#endif
+}
+
 // A copy of x the compiler must treat as unknown: stops it from hoisting
 // per-channel texel extractions out of the candidate loops and keeping 48
 // unpacked values live; re-extracting a byte costs one instruction.
