@@ -336,7 +336,7 @@ BCN_HD void bc6h_qd(const Bc6hState& st, const float e[3], int q[3], int d[3]) {
   const int levels = 1 << st.epb;
 #pragma unroll
   for (int p = 0; p < 3; p++) {
-    int v = qclamp(e[p] / (256.0f * 256.0f - 1.0f) * (float)(levels - 1) + 0.5f, 0.0f,
+    int v = qclamp(div65535(e[p]) * (float)(levels - 1) + 0.5f, 0.0f,
                    (float)(levels - 1));
     v = iclamp(v, st.qlo[p], st.qhi[p]);
     q[p] = v;
@@ -353,9 +353,9 @@ BCN_HD void bc6h_qbounds(Bc6hState& st, const float span[3]) {
     float lo = middle - span[p] / 2.0f;
     float hi = middle + span[p] / 2.0f;
     // ep_quant_bc6h_8 (bc6h.rs:166-173)
-    st.qlo[p] = qclamp(lo / (256.0f * 256.0f - 1.0f) * (float)(levels - 1) + 0.5f, 0.0f,
+    st.qlo[p] = qclamp(div65535(lo) * (float)(levels - 1) + 0.5f, 0.0f,
                        (float)(levels - 1));
-    st.qhi[p] = qclamp(hi / (256.0f * 256.0f - 1.0f) * (float)(levels - 1) + 0.5f, 0.0f,
+    st.qhi[p] = qclamp(div65535(hi) * (float)(levels - 1) + 0.5f, 0.0f,
                        (float)(levels - 1));
   }
 }

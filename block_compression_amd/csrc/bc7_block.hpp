@@ -131,22 +131,24 @@ BCN_HD Moments moments(const uint32_t px[16], uint32_t mask, uint32_t sh = 0) {
   return m;
 }
 
-// covar_from_stats (common.rs:334-350), f32 with the reference's order.
+// covar_from_stats (common.rs:334-350), f32 with the reference's order.  The
+// products of sums are integers <= 4080^2 < 2^24 and n is in [1, 16], so
+// x / n is div_rcp with rn = 1.0f / n.
 template <int CH>
-BCN_HD void covar_from_moments(float c[10], const int s[4], const int q[10], int n) {
+BCN_HD void covar_from_moments(float c[10], const int s[4], const int q[10], int n, float rn) {
   float s14 = (float)n, s10 = (float)s[0], s11 = (float)s[1], s12 = (float)s[2];
-  c[0] = (float)q[0] - s10 * s10 / s14;
-  c[1] = (float)q[1] - s10 * s11 / s14;
-  c[2] = (float)q[2] - s10 * s12 / s14;
-  c[4] = (float)q[4] - s11 * s11 / s14;
-  c[5] = (float)q[5] - s11 * s12 / s14;
-  c[7] = (float)q[7] - s12 * s12 / s14;
+  c[0] = (float)q[0] - div_rcp(s10 * s10, s14, rn);
+  c[1] = (float)q[1] - div_rcp(s10 * s11, s14, rn);
+  c[2] = (float)q[2] - div_rcp(s10 * s12, s14, rn);
+  c[4] = (float)q[4] - div_rcp(s11 * s11, s14, rn);
+  c[5] = (float)q[5] - div_rcp(s11 * s12, s14, rn);
+  c[7] = (float)q[7] - div_rcp(s12 * s12, s14, rn);
   if (CH == 4) {
     float s13 = (float)s[3];
-    c[3] = (float)q[3] - s10 * s13 / s14;
-    c[6] = (float)q[6] - s11 * s13 / s14;
-    c[8] = (float)q[8] - s12 * s13 / s14;
-    c[9] = (float)q[9] - s13 * s13 / s14;
+    c[3] = (float)q[3] - div_rcp(s10 * s13, s14, rn);
+    c[6] = (float)q[6] - div_rcp(s11 * s13, s14, rn);
+    c[8] = (float)q[8] - div_rcp(s12 * s13, s14, rn);
+    c[9] = (float)q[9] - div_rcp(s13 * s13, s14, rn);
   } else {
     c[3] = c[6] = c[8] = c[9] = 0.0f;
   }
@@ -158,12 +160,12 @@ template <int CH, int ROT = 0>
 BCN_HD void pca_segment(const uint32_t px[16], uint32_t mask, float e0[4], float e1[4],
                         uint32_t sh = 0) {
   Moments m = moments<CH, ROT>(px, mask, sh);
-  float s14 = (float)m.n;
+  float s14 = (float)m.n, rn = 1.0f / s14;
   float dc[4];
 #pragma unroll
-  for (int p = 0; p < 4; p++) dc[p] = p < CH ? (float)m.s[p] / s14 : 0.0f;
+  for (int p = 0; p < 4; p++) dc[p] = p < CH ? div_rcp((float)m.s[p], s14, rn) : 0.0f;
   float c[10];
-  covar_from_moments<CH>(c, m.s, m.q, m.n);
+  covar_from_moments<CH>(c, m.s, m.q, m.n, rn);
   const float inv_var = 1.0f / (256.0f * 256.0f);
   const float eps = 0.001f * 0.001f;
 #pragma unroll
@@ -238,7 +240,7 @@ BCN_HD void quant_pair_0367(const float e0[4], const float e1[4], int q0[4], int
     for (int b = 0; b < 2; b++)
 #pragma unroll
       for (int p = 0; p < QCH; p++) {
-        float x = (ep[p] / 255.0f * (float)L2 - (float)b) / 2.0f + 0.5f;
+        float x = (div255(ep[p]) * (float)L2 - (float)b) / 2.0f + 0.5f;
         qb[b][p] = qclamp(x, 0.0f, (float)(L - 1)) * 2 + b;
         eb[b][p] = MODE == 0 ? (float)unpack_to_byte(qb[b][p], 5) : (float)qb[b][p];
       }
@@ -273,7 +275,7 @@ BCN_HD void quant_pair_1(const float e0[4], const float e1[4], int q0[4], int q1
 #pragma unroll
       for (int p = 0; p < 3; p++) {
         float v = (i == 0 ? e0[p] : e1[p]);
-        float x = (v / 255.0f * 127.0f - (float)b) / 2.0f + 0.5f;
+        float x = (div255(v) * 127.0f - (float)b) / 2.0f + 0.5f;
         qb[b][i][p] = qclamp(x, 0.0f, 63.0f) * 2 + b;
         eb[b][i][p] = (float)unpack_to_byte(qb[b][i][p], 7);
       }
@@ -304,8 +306,8 @@ BCN_HD void quant_pair_245(const float e0[4], const float e1[4], int q0[4], int 
   constexpr int L = 1 << BITS;
 #pragma unroll
   for (int p = 0; p < 3; p++) {
-    q0[p] = qclamp(e0[p] / 255.0f * (float)(L - 1) + 0.5f, 0.0f, (float)(L - 1));
-    q1[p] = qclamp(e1[p] / 255.0f * (float)(L - 1) + 0.5f, 0.0f, (float)(L - 1));
+    q0[p] = qclamp(div255(e0[p]) * (float)(L - 1) + 0.5f, 0.0f, (float)(L - 1));
+    q1[p] = qclamp(div255(e1[p]) * (float)(L - 1) + 0.5f, 0.0f, (float)(L - 1));
     d0[p] = unpack_to_byte(q0[p], BITS);
     d1[p] = unpack_to_byte(q1[p], BITS);
   }
@@ -332,7 +334,7 @@ BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pat
                        uint32_t qblock[2], uint32_t sh = 0) {
   constexpr int L = 1 << BITS;
   int dd[3][4] = {};
-  float fdiv[3] = {1.0f, 1.0f, 1.0f};
+  float fdiv[3] = {1.0f, 1.0f, 1.0f}, frcp[3] = {1.0f, 1.0f, 1.0f};
 #pragma unroll
   for (int j = 0; j < NSUB; j++) {
     int div = 0;
@@ -342,6 +344,8 @@ BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pat
       div += mul24(dd[j][p], dd[j][p]);
     }
     fdiv[j] = (float)div;
+    // div == 0 gives +inf here and NaN below (proj is then 0), as 0 / 0 does.
+    frcp[j] = 1.0f / fdiv[j];
   }
   int total = 0;
   qblock[0] = 0;
@@ -350,18 +354,20 @@ BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pat
   for (int k = 0; k < 16; k++) {
     int j = NSUB == 1 ? 0 : (int)((pattern >> (2 * k)) & 3u);
     int ea[4], de[4];
-    float div;
+    float div, rdiv;
 #pragma unroll
     for (int p = 0; p < CH; p++) {
       ea[p] = NSUB == 1 ? d[0][0][p] : (j == 0 ? d[0][0][p] : (j == 1 ? d[1][0][p] : d[2][0][p]));
       de[p] = NSUB == 1 ? dd[0][p] : (j == 0 ? dd[0][p] : (j == 1 ? dd[1][p] : dd[2][p]));
     }
     div = NSUB == 1 ? fdiv[0] : (j == 0 ? fdiv[0] : (j == 1 ? fdiv[1] : fdiv[2]));
+    rdiv = NSUB == 1 ? frcp[0] : (j == 0 ? frcp[0] : (j == 1 ? frcp[1] : frcp[2]));
     uint32_t t = texel<ROT>(px, k, sh);
     int proj = 0;
 #pragma unroll
     for (int p = 0; p < CH; p++) proj += mul24(chi(t, p) - ea[p], de[p]);
-    float fp = (float)proj / div;
+    // |proj| <= |texel - e0| |e1 - e0| <= 510 sqrt(div) (Cauchy-Schwarz).
+    float fp = div_rcp((float)proj, div, rdiv);
     int q1 = qclamp(fp * (float)L + 0.5f, 1.0f, (float)(L - 1));
     int w0 = weight<BITS>(q1 - 1), w1 = weight<BITS>(q1);
     int err0 = 0, err1 = 0;
@@ -416,6 +422,7 @@ BCN_HD void opt_endpoints(const uint32_t px[16], const uint32_t qblock[2], uint3
   int det = mul24i(cxx, cyy) - mul24i(cxy, cxy);
   float scale = (float)L1 / (float)det;
   float fcxx = (float)cxx, fcyy = (float)cyy, fcxy = (float)cxy;
+  float fcnt = (float)cnt, rcnt = 1.0f / fcnt;
 #pragma unroll
   for (int p = 0; p < CH; p++) {
     float a1 = (float)atb1[p];
@@ -423,7 +430,7 @@ BCN_HD void opt_endpoints(const uint32_t px[16], const uint32_t qblock[2], uint3
     float x0 = (a1 * fcyy - a2 * fcxy) * scale;
     float x1 = (a2 * fcxx - a1 * fcxy) * scale;
     // |det| < 0.001 with an integer det: flatten to the mean.
-    float mean = (float)sum[p] / (float)cnt;
+    float mean = div_rcp((float)sum[p], fcnt, rcnt);  // sum <= 4080, cnt in [1, 16]
     e0[p] = det == 0 ? mean : x0;
     e1[p] = det == 0 ? mean : x1;
   }
@@ -446,14 +453,14 @@ BCN_HD int chan(const uint32_t px[16], int k, uint32_t sh) {
 template <int BITS>
 BCN_HD int channel_quant(const uint32_t px[16], uint32_t sh, int e0, int e1, uint32_t qblock[2]) {
   constexpr int L = 1 << BITS;
-  float den = (float)(e1 - e0) + 0.001f;
+  float den = (float)(e1 - e0) + 0.001f, rden = 1.0f / den;
   qblock[0] = 0;
   qblock[1] = 0;
   int total = 0;
 #pragma unroll
   for (int k = 0; k < 16; k++) {
     const int cvk = chan(px, k, sh);
-    float proj = (float)(cvk - e0) / den;
+    float proj = div_rcp((float)(cvk - e0), den, rden);
     int q1 = qclamp(proj * (float)L + 0.5f, 1.0f, (float)(L - 1));
     int w0 = weight<BITS>(q1 - 1), w1 = weight<BITS>(q1);
     // Interpolation as in block_quant: a + ((w * (b - a) + 32) >> 6).
@@ -471,7 +478,7 @@ BCN_HD int channel_quant(const uint32_t px[16], uint32_t sh, int e0, int e1, uin
 template <int EPBITS>
 BCN_HD int channel_qd(float e, int& q) {
   constexpr int E = 1 << EPBITS;
-  q = qclamp(e / 255.0f * (float)(E - 1) + 0.5f, 0.0f, (float)(E - 1));
+  q = qclamp(div255(e) * (float)(E - 1) + 0.5f, 0.0f, (float)(E - 1));
   return unpack_to_byte(q, EPBITS);
 }
 
@@ -676,7 +683,7 @@ BCN_HD void partition_bounds(const uint32_t px[16], uint16_t* keys, int stride) 
         for (int i = 0; i < 10; i++) q[i] = full.q[i] - m.q[i];
       }
       float c[10];
-      covar_from_moments<CH>(c, sm, q, n);
+      covar_from_moments<CH>(c, sm, q, n, 1.0f / (float)n);
       // get_pca_bound (common.rs:208-245).
       const float inv_var = 1.0f / (256.0f * 256.0f);
       const float eps = 0.001f * 0.001f;

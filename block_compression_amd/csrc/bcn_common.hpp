@@ -67,6 +67,28 @@ BCN_HD int mul24i(int a, int b) {
 #endif
 }
 
+// a / b from y = 1.0f / b (Markstein's correction step): q = a*y, then the
+// exact residual r = a - b*q by FMA, then q + r*y rounded once.  This is not
+// correctly rounded for every (a, b), so it is used only at call sites whose
+// whole input domain was checked exhaustively against IEEE division
+// (tests/divcheck/divcheck.cpp), namely
+//   * integer a in [0, 2^24] over integer b in [1, 16];
+//   * integer a with |a| <= 510 sqrt(b) over integer b in [1, 260100];
+//   * integer a in [-255, 255] over b = k + 0.001f, integer k in [-255, 255];
+//   * every finite a over b = 255 or b = 65535, except that a -0 numerator
+//     gives +0; each of those sites adds 0.5 (and subtracts a p-bit) after
+//     scaling the quotient, which maps -0 and +0 to the same value.
+// Three operations instead of the ten of a correctly rounded division.
+BCN_HD float div_rcp(float a, float b, float y) {
+  float q = a * y;
+  float r = __builtin_fmaf(-b, q, a);
+  return __builtin_fmaf(r, y, q);
+}
+
+// x / 255.0f and x / 65535.0f for every finite x (see div_rcp).
+BCN_HD float div255(float x) { return div_rcp(x, 255.0f, 1.0f / 255.0f); }
+BCN_HD float div65535(float x) { return div_rcp(x, 65535.0f, 1.0f / 65535.0f); }
+
 // A copy of x the compiler must treat as unknown: stops it from hoisting
 // per-channel texel extractions out of the candidate loops and keeping 48
 // unpacked values live; re-extracting a byte costs one instruction.
