@@ -55,6 +55,106 @@ def psnr_rgb(orig, dec) -> float:
     return 0.0 if mse == 0.0 else 20.0 * math.log10(1.0 / math.sqrt(mse))
 
 
+def bench_batched(n_textures: int, reps: int, oracle_levels: int, threads: int) -> dict:
+    """The batched mixed mip-chain workload (block_compression_amd.workloads):
+    the same tasks encoded by per-task launches (bcn_compress, one kernel per
+    mip level, as the reference dispatches) and by one batched persistent
+    launch (bcn_batch, autotuned, hipGraph).  Both outputs must be identical;
+    a seeded sample of levels is checked against the oracle."""
+    import numpy as np
+    import torch
+
+    from block_compression_amd import GpuBlockCompressor, workloads
+
+    t0 = time.perf_counter()
+    levels = workloads.build(n_textures, seed=4, device="cuda")
+    torch.cuda.synchronize()
+    gen_s = time.perf_counter() - t0
+    total = workloads.output_bytes(levels)
+    texels = sum(lv.width * lv.height for lv in levels)
+    blocks = texels // 16
+    log(f"bench: batched workload {n_textures} textures, {len(levels)} levels, "
+        f"{texels / 1e9:.3f} Gpx, generated in {gen_s:.1f} s")
+    plain = torch.zeros(total, dtype=torch.uint8, device="cuda")
+    batched = torch.zeros(total, dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+    comp = GpuBlockCompressor(0)
+
+    def timed(fn, n):
+        best = None
+        for _ in range(n):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            a.record(stream)
+            fn()
+            b.record(stream)
+            torch.cuda.synchronize()
+            ms = a.elapsed_time(b)
+            best = ms if best is None else min(best, ms)
+        return best
+
+    def plain_step():
+        workloads.queue(comp, levels, plain)
+        comp.compress()
+
+    plain_step()
+    torch.cuda.synchronize()
+    plain_ms = timed(plain_step, reps)
+    log(f"bench: per-task launches {plain_ms:.2f} ms")
+    workloads.queue(comp, levels, batched)
+    batch = comp.create_batch()
+    cfg = batch.autotune(reps=1)
+    batch.launch(graph=True)
+    torch.cuda.synchronize()
+    batched_ms = timed(lambda: batch.launch(graph=True), reps)
+    info = batch.info()
+    log(f"bench: batched persistent launch {batched_ms:.2f} ms, config {cfg}")
+    same = bool(torch.equal(plain, batched))
+    # Oracle on a seeded sample of levels (every level of the sample, all blocks).
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+
+    rng = np.random.default_rng(5)
+    small = [i for i, lv in enumerate(levels) if lv.width * lv.height <= 64 * 64]
+    big = [i for i, lv in enumerate(levels) if lv.width * lv.height > 64 * 64]
+    pick = list(rng.choice(small, min(oracle_levels, len(small)), replace=False))
+    pick += list(rng.choice(big, min(max(1, oracle_levels // 16), len(big)), replace=False))
+    out = batched.cpu().numpy()
+    checked = identical = 0
+    for i in pick:
+        lv = levels[int(i)]
+        v = workloads.variant_of(lv.fmt)
+        img = lv.texels.cpu().numpy()
+        nb = (lv.width // 4) * (lv.height // 4)
+        ids = np.arange(nb) if nb <= 4096 else np.sort(rng.choice(nb, 4096, replace=False))
+        ref = oracle_lib.compress_rgba8_sample(v.tag, v.settings_bytes(), img, ids, threads)
+        bb = v.block_byte_size()
+        got = out[lv.offset:lv.offset + lv.nbytes].reshape(-1, bb)[ids]
+        identical += int((got == ref.reshape(-1, bb)).all(axis=1).sum())
+        checked += len(ids)
+    batch.close()
+    return {
+        "workload": f"{n_textures} tile-generator RGBA8 textures, w,h ~ U{{64..4096}}, full box-filtered "
+                    "mip chains padded to 4, formats ~ U{BC1,BC3,BC4,BC5,BC7 opaque_basic,BC7 alpha_basic}",
+        "levels": len(levels),
+        "texels": int(texels),
+        "blocks": int(blocks),
+        "plain_ms": round(plain_ms, 3),
+        "plain_mpix_s": round(texels / (plain_ms * 1e-3) / 1e6, 1),
+        "batched_ms": round(batched_ms, 3),
+        "batched_mpix_s": round(texels / (batched_ms * 1e-3) / 1e6, 1),
+        "speedup_vs_plain": round(plain_ms / batched_ms, 3),
+        "autotuned_config": cfg,
+        "grid": info["grid"],
+        "chunks": info["chunks"],
+        "identical_to_plain": same,
+        "oracle_blocks_checked": checked,
+        "oracle_identical_frac": round(identical / checked, 6) if checked else None,
+        "timing": f"best of {reps} launches each, HIP events on one stream; plain = one kernel per "
+                  "mip level (bcn_compress), batched = bcn_batch_launch_graph",
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -63,6 +163,9 @@ def main() -> int:
     ap.add_argument("--parity-blocks", type=int, default=65536,
                     help="seeded sample of blocks checked against the CPU oracle (0 = none)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0 = min(16, cores))")
+    ap.add_argument("--batched-textures", type=int, default=300,
+                    help="textures in the batched mip-chain workload (0 = skip it)")
+    ap.add_argument("--batched-reps", type=int, default=3)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times "
                          "(reference kernel for scaling HBM byte counters) and exit")
@@ -206,6 +309,10 @@ def main() -> int:
             "hbm_frac": round(nblocks * 80 / (kernel_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 5),
         }
 
+    batched = None
+    if args.batched_textures > 0:
+        batched = bench_batched(args.batched_textures, args.batched_reps, 24, threads)
+
     line = {
         "metric": METRIC,
         "value": round(mpix, 2),
@@ -232,6 +339,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu_baseline,
         "parity": parity,
+        "batched": batched,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -13,6 +13,7 @@ from .settings import (  # noqa: F401
     CompressionVariant,
 )
 from .compressor import (  # noqa: F401
+    BlockCompressionBatch,
     GpuBlockCompressor,
     compress_rgba8,
     decode_bc6h_rgba16f,
@@ -26,6 +27,7 @@ __all__ = [
     "BC6H_PRESETS",
     "BC7_PRESETS",
     "CompressionVariant",
+    "BlockCompressionBatch",
     "GpuBlockCompressor",
     "BcnError",
     "compress_rgba8",

@@ -46,7 +46,30 @@ SIGNATURES = [
     ("bcn_decode_rgba8", _i32, [_i32, _vp, _sz, _u32, _u32, _vp, _u32, _vp]),
     ("bcn_decode_bc6h_rgba16f", _i32, [_vp, _sz, _u32, _u32, _i32, _vp, _u32, _vp]),
     ("bcn_partition_tables", None, [_vp, _vp, _vp, _vp]),
+    ("bcn_batch_create", _i32, [_vp, _vp, ctypes.POINTER(_vp)]),
+    ("bcn_batch_launch", _i32, [_vp, _vp]),
+    ("bcn_batch_launch_graph", _i32, [_vp, _vp]),
+    ("bcn_batch_autotune", _i32, [_vp, _vp, _i32, _vp]),
+    ("bcn_batch_get_info", _i32, [_vp, _vp]),
+    ("bcn_batch_last_error", ctypes.c_char_p, [_vp]),
+    ("bcn_batch_destroy", None, [_vp]),
 ]
+
+
+class BatchConfig(ctypes.Structure):
+    """bcn_batch_config."""
+
+    _fields_ = [("chunk_blocks_bc7", _u32), ("chunk_blocks_bc15", _u32), ("waves_per_cu", _u32)]
+
+    def as_dict(self) -> dict:
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+class BatchInfo(ctypes.Structure):
+    """bcn_batch_info."""
+
+    _fields_ = [("tasks", _u32), ("persistent_tasks", _u32), ("blocks", ctypes.c_uint64),
+                ("chunks", _u32), ("grid", _u32), ("config", BatchConfig)]
 
 
 class BcnError(RuntimeError):
@@ -78,9 +101,11 @@ def lib() -> ctypes.CDLL:
     return _lib
 
 
-def check(status: int, handle=None) -> None:
+def check(status: int, handle=None, batch=None) -> None:
     if status != BCN_OK:
         msg = ""
         if handle is not None:
             msg = (lib().bcn_last_error(handle) or b"").decode()
+        elif batch is not None:
+            msg = (lib().bcn_batch_last_error(batch) or b"").decode()
         raise BcnError(status, msg)

@@ -118,6 +118,70 @@ class GpuBlockCompressor:
         _capi.check(status, self._handle)
 
 
+    def create_batch(self, config: Optional[dict] = None) -> "BlockCompressionBatch":
+        """Moves every queued task into a BlockCompressionBatch (one persistent
+        launch over all of them; see bcn_batch_create)."""
+        return BlockCompressionBatch(self, config)
+
+
+def _stream_handle(device, stream) -> int:
+    if stream is None:
+        return _current_stream_handle(device)
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+class BlockCompressionBatch:
+    """Every task queued on a GpuBlockCompressor, encoded by one persistent
+    kernel over a block work queue (no counterpart in the reference, which
+    dispatches per task).  launch() may be called any number of times; the
+    textures and buffers of the tasks must stay alive while the batch does."""
+
+    def __init__(self, compressor: GpuBlockCompressor, config: Optional[dict] = None):
+        self.device = compressor.device
+        cfg = None
+        if config is not None:
+            cfg = _capi.BatchConfig(**config)
+        handle = ctypes.c_void_p()
+        status = _capi.lib().bcn_batch_create(
+            compressor._handle, ctypes.byref(cfg) if cfg is not None else None, ctypes.byref(handle))
+        _capi.check(status, compressor._handle)
+        self._handle = handle
+        self._keepalive = list(compressor._keepalive)
+        compressor._keepalive.clear()
+
+    def launch(self, stream=None, graph: bool = False) -> None:
+        fn = _capi.lib().bcn_batch_launch_graph if graph else _capi.lib().bcn_batch_launch
+        _capi.check(fn(self._handle, ctypes.c_void_p(_stream_handle(self.device, stream))),
+                    batch=self._handle)
+
+    def autotune(self, reps: int = 2, stream=None) -> dict:
+        cfg = _capi.BatchConfig()
+        _capi.check(_capi.lib().bcn_batch_autotune(
+            self._handle, ctypes.c_void_p(_stream_handle(self.device, stream)), reps,
+            ctypes.byref(cfg)), batch=self._handle)
+        return cfg.as_dict()
+
+    def info(self) -> dict:
+        inf = _capi.BatchInfo()
+        _capi.check(_capi.lib().bcn_batch_get_info(self._handle, ctypes.byref(inf)), batch=self._handle)
+        return {"tasks": inf.tasks, "persistent_tasks": inf.persistent_tasks, "blocks": inf.blocks,
+                "chunks": inf.chunks, "grid": inf.grid, "config": inf.config.as_dict()}
+
+    def close(self) -> None:
+        if getattr(self, "_handle", None):
+            _capi.lib().bcn_batch_destroy(self._handle)
+            self._handle = None
+            self._keepalive = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def compress_rgba8(variant: CompressionVariant, texture, stream=None):
     """One-shot helper: encode a whole (H, W, 4) device texture, return the
     block buffer (a uint8 device tensor of variant.blocks_byte_size bytes)."""

@@ -25,6 +25,15 @@
 #pragma once
 #include "bcn_common.hpp"
 
+// Occupancy target of kernels running the BC7 search, in waves per SIMD.
+// Unconstrained, the block search takes about 240 VGPRs (2 waves); at 3 waves
+// it fits 168 VGPRs with a few bytes per lane spilled.  On 4096x4096
+// opaque_basic, 2/3/4 waves ran 7.04/6.91/6.95 ms per launch.  The 8 KiB of
+// LDS per 64-lane workgroup allows 5.
+#ifndef BCN_BC7_WAVES
+#define BCN_BC7_WAVES 3
+#endif
+
 namespace bcn {
 
 // BC7Settings, 19 x u32 in the reference's repr(C) order (src/settings.rs:80-90).

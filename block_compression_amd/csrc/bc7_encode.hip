@@ -14,13 +14,6 @@ namespace bcn {
 
 constexpr int kBc7Wave = 64;
 
-// Occupancy target of the BC7 kernel in waves per SIMD.  Unconstrained, the
-// block search takes about 240 VGPRs (2 waves); at 3 waves it fits 168 VGPRs
-// with 16 bytes per lane spilled.  On 4096x4096 opaque_basic, 2/3/4 waves ran
-// 7.04/6.91/6.95 ms per launch.  The 8 KiB of LDS per workgroup allows 5.
-#ifndef BCN_BC7_WAVES
-#define BCN_BC7_WAVES 3
-#endif
 
 template <int CH, bool VEC>
 __global__ __launch_bounds__(kBc7Wave)

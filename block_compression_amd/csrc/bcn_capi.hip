@@ -10,12 +10,6 @@
 #include "bcn_common.hpp"
 #include "bcn_internal.hpp"
 
-struct bcn_compressor {
-  int device;
-  std::vector<bcn::Task> tasks;
-  std::string last_error;
-};
-
 namespace {
 
 int fail(bcn_compressor* c, int status, const char* what) {
@@ -27,21 +21,15 @@ uint32_t block_bytes(bcn_variant v) { return (v == BCN_BC1 || v == BCN_BC4) ? 8u
 
 bool valid_variant(int v) { return v >= BCN_BC1 && v <= BCN_BC7; }
 
-// Restores the caller's current device on scope exit.
-struct DeviceGuard {
-  int prev = -1;
-  bool ok = false;
-  explicit DeviceGuard(int dev) {
-    if (hipGetDevice(&prev) != hipSuccess) return;
-    ok = prev == dev || hipSetDevice(dev) == hipSuccess;
-  }
-  ~DeviceGuard() {
-    int cur = -1;
-    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
-  }
-};
-
 }  // namespace
+
+hipError_t bcn::launch_task(const bcn::Task& t, hipStream_t stream) {
+  switch (t.variant) {
+    case BCN_BC7: return bcn::launch_bc7(t, stream);
+    case BCN_BC6H: return bcn::launch_bc6h(t, stream);
+    default: return bcn::launch_bc15(t, stream);
+  }
+}
 
 extern "C" {
 
@@ -161,16 +149,11 @@ int bcn_compress(bcn_compressor* c, void* hip_stream) {
   if (!c) return BCN_ERR_INVALID_ARGUMENT;
   std::vector<bcn::Task> tasks;
   tasks.swap(c->tasks);  // the queue is drained whatever happens (block_compressor.rs:550)
-  DeviceGuard guard(c->device);
+  bcn::DeviceGuard guard(c->device);
   if (!guard.ok) return fail(c, BCN_ERR_HIP, "hipSetDevice failed");
   hipStream_t stream = static_cast<hipStream_t>(hip_stream);
   for (const bcn::Task& t : tasks) {
-    hipError_t e = hipSuccess;
-    switch (t.variant) {
-      case BCN_BC7: e = bcn::launch_bc7(t, stream); break;
-      case BCN_BC6H: e = bcn::launch_bc6h(t, stream); break;
-      default: e = bcn::launch_bc15(t, stream); break;
-    }
+    hipError_t e = bcn::launch_task(t, stream);
     if (e == hipErrorNotSupported) return fail(c, BCN_ERR_UNSUPPORTED, "format not built in this library");
     if (e != hipSuccess) {
       char buf[256];

@@ -3,6 +3,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+#include <vector>
+
 #include "../../include/bcn_mi355x.h"
 
 namespace bcn {
@@ -25,4 +28,28 @@ hipError_t launch_bc7(const Task& t, hipStream_t stream);
 hipError_t launch_bc15(const Task& t, hipStream_t stream);
 hipError_t launch_bc6h(const Task& t, hipStream_t stream);
 
+// Launches one task with the per-format kernel.
+hipError_t launch_task(const Task& t, hipStream_t stream);
+
+// Restores the caller's current device on scope exit.
+struct DeviceGuard {
+  int prev = -1;
+  bool ok = false;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) return;
+    ok = prev == dev || hipSetDevice(dev) == hipSuccess;
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
 }  // namespace bcn
+
+// The task queue behind the C ABI's bcn_compressor handle.
+struct bcn_compressor {
+  int device;
+  std::vector<bcn::Task> tasks;
+  std::string last_error;
+};

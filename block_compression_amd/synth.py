@@ -201,3 +201,124 @@ def hdr_image_rgba16f(width: int, height: int, seed: int = 3) -> np.ndarray:
 
 def sha256(a: np.ndarray) -> str:
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+# ---- the tile generator on a torch device -------------------------------------------
+#
+# The same arithmetic as tile_image_rgba8 on torch tensors (float64 for the
+# tile maths), so that large workloads can be generated where they are used.
+# torch has no uint64 arithmetic: hashes are int64 bit patterns (addition and
+# multiplication wrap like uint64; right shifts are masked to be logical).
+
+
+def _s64(c: int) -> int:
+    c &= (1 << 64) - 1
+    return c - (1 << 64) if c >= (1 << 63) else c
+
+
+def _shr(z, k: int):
+    return (z >> k) & ((1 << (64 - k)) - 1)
+
+
+def splitmix64_t(x):
+    """splitmix64 on an int64 tensor holding uint64 bit patterns."""
+    z = x + _s64(0x9E3779B97F4A7C15)
+    z = (z ^ _shr(z, 30)) * _s64(0xBF58476D1CE4E5B9)
+    z = (z ^ _shr(z, 27)) * _s64(0x94D049BB133111EB)
+    return z ^ _shr(z, 31)
+
+
+def _draws_t(seed: int, tile, purpose: int, n: int):
+    import torch
+
+    base = int(splitmix64(np.uint64(seed) * np.uint64(0x100000001B3) + np.uint64(purpose)))
+    k = splitmix64_t(_s64(base) ^ (tile * _s64(0xD6E8FEB86659FD93)))
+    h = splitmix64_t(k[:, None] + torch.arange(n, dtype=torch.int64, device=tile.device)[None, :])
+    return _shr(h, 40).to(torch.float64) / float(1 << 24)
+
+
+def _value_noise_t(lattice, freq: int, coord):
+    import torch
+
+    g = coord * freq
+    i = torch.floor(g).to(torch.int64)
+    t = g - i
+    l00 = lattice[:, :, i[:, None], i[None, :]]
+    l10 = lattice[:, :, i[:, None], i[None, :] + 1]
+    l01 = lattice[:, :, i[:, None] + 1, i[None, :]]
+    l11 = lattice[:, :, i[:, None] + 1, i[None, :] + 1]
+    top = l00 + (l10 - l00) * t[None, None, None, :]
+    bot = l01 + (l11 - l01) * t[None, None, None, :]
+    return top + (bot - top) * t[None, None, :, None]
+
+
+def tile_image_rgba8_torch(width: int, height: int, seed: int = 1, noise_frac: float = 0.0,
+                           device="cuda"):
+    """tile_image_rgba8 computed with torch on `device`: a (height, width, 4)
+    uint8 tensor.  Equal to the numpy version up to the last-ulp behaviour of
+    the device's sin/cos on edge tiles."""
+    import math
+
+    import torch
+
+    if width % 8 or height % 8:
+        raise ValueError("width and height must be multiples of 8")
+    tw, th = width // 8, height // 8
+    nt = tw * th
+    f64 = torch.float64
+    tiles = torch.arange(nt, dtype=torch.int64, device=device)
+    kind_u = _draws_t(seed, tiles, 1, 2)
+    u = kind_u[:, 1]
+    kind = torch.where(u < 0.5, 0, torch.where(u < 0.8, 1, 2))
+    kind = torch.where(kind_u[:, 0] < noise_frac, 3, kind)
+    out = torch.zeros((nt, 8, 8, 3), dtype=f64, device=device)
+    coord = (torch.arange(8, dtype=f64, device=device) + 0.5) / 8.0
+    lin = torch.arange(8, dtype=f64, device=device) / 7.0
+
+    sel = torch.nonzero(kind == 0).flatten()
+    if sel.numel():
+        c = _draws_t(seed, sel, 2, 12).reshape(-1, 4, 3) * 255.0
+        fx = lin[None, None, :, None]
+        fy = lin[None, :, None, None]
+        top = c[:, None, None, 0] + (c[:, None, None, 1] - c[:, None, None, 0]) * fx
+        bot = c[:, None, None, 2] + (c[:, None, None, 3] - c[:, None, None, 2]) * fx
+        out[sel] = top + (bot - top) * fy
+
+    sel = torch.nonzero(kind == 1).flatten()
+    if sel.numel():
+        par = _draws_t(seed, sel, 3, 4)
+        base = par[:, :3] * 255.0
+        amp = par[:, 3] * 255.0
+        acc = torch.zeros((sel.numel(), 3, 8, 8), dtype=f64, device=device)
+        norm = 0.0
+        for o, f in enumerate((1, 2, 4, 8)):
+            lat = _draws_t(seed, sel, 10 + o, 3 * (f + 1) * (f + 1)).reshape(-1, 3, f + 1, f + 1)
+            w = 0.5 ** o
+            acc += w * _value_noise_t(lat, f, coord)
+            norm += w
+        n = acc / norm - 0.5
+        out[sel] = (base[:, :, None, None] + amp[:, None, None, None] * n).permute(0, 2, 3, 1)
+
+    sel = torch.nonzero(kind == 2).flatten()
+    if sel.numel():
+        par = _draws_t(seed, sel, 4, 9)
+        c0 = par[:, 0:3] * 255.0
+        c1 = par[:, 3:6] * 255.0
+        px = par[:, 6] * 8.0
+        py = par[:, 7] * 8.0
+        th_ = par[:, 8] * math.pi
+        xs = torch.arange(8, dtype=f64, device=device) + 0.5
+        d = (xs[None, None, :] - px[:, None, None]) * torch.sin(th_)[:, None, None] - (
+            xs[None, :, None] - py[:, None, None]) * torch.cos(th_)[:, None, None]
+        side = (d >= 0.0)[..., None]
+        out[sel] = torch.where(side, c1[:, None, None, :], c0[:, None, None, :])
+
+    sel = torch.nonzero(kind == 3).flatten()
+    if sel.numel():
+        out[sel] = _draws_t(seed, sel, 5, 192).reshape(-1, 8, 8, 3) * 256.0
+
+    rgb = torch.clamp(torch.floor(out + 0.5), 0, 255).to(torch.uint8)
+    img = torch.empty((th, 8, tw, 8, 4), dtype=torch.uint8, device=device)
+    img[..., :3] = rgb.reshape(th, tw, 8, 8, 3).permute(0, 2, 1, 3, 4)
+    img[..., 3] = 255
+    return img.reshape(height, width, 4)

@@ -148,6 +148,63 @@ int bcn_decode_bc6h_rgba16f(const void* blocks, size_t blocks_size, uint32_t wid
 void bcn_partition_tables(uint32_t* pattern, uint32_t* mask01, uint8_t* anchor1,
                           uint8_t* anchor2);
 
+/* ---- Batched persistent dispatch ---------------------------------------------------
+ *
+ * No counterpart in the reference, which records one compute dispatch per
+ * queued task (GpuBlockCompressor::compress, src/block_compressor.rs:540-590).
+ * A batch takes every task queued on a compressor -- any number of textures
+ * and mip levels, mixed formats -- and encodes the BC1-BC5 and BC7 ones in ONE
+ * persistent kernel over a device-resident block work queue (chunks of blocks
+ * of one task, most expensive first); BC6H tasks of the batch are launched
+ * per task after it.  The output bytes are the ones bcn_compress writes. */
+typedef struct bcn_batch bcn_batch;
+
+typedef struct bcn_batch_config {
+  uint32_t chunk_blocks_bc7;   /* blocks per work item for BC7 tasks (multiple of 64)    */
+  uint32_t chunk_blocks_bc15;  /* blocks per work item for BC1-BC5 tasks (multiple of 64) */
+  uint32_t waves_per_cu;       /* persistent grid = CUs x this, one 64-lane wave per
+                                  workgroup; 0 = as many as can be resident           */
+} bcn_batch_config;
+
+typedef struct bcn_batch_info {
+  uint32_t tasks;              /* tasks in the batch (all formats)       */
+  uint32_t persistent_tasks;   /* of which BC1-BC5 / BC7 (work queue)    */
+  uint64_t blocks;             /* blocks in the work queue               */
+  uint32_t chunks;             /* work items in the queue                */
+  uint32_t grid;               /* workgroups of the persistent launch    */
+  bcn_batch_config config;     /* configuration in use                   */
+} bcn_batch_info;
+
+/* Moves every task queued on `c` into a new batch and uploads its work queue
+ * (the only allocation and host-to-device copy).  config NULL = defaults.
+ * The textures and buffers the tasks name must outlive the batch. */
+int bcn_batch_create(bcn_compressor* c, const bcn_batch_config* config, bcn_batch** out);
+
+/* Enqueues the batch on hip_stream: a one-thread kernel that rewinds the
+ * queue, the persistent kernel, then any BC6H tasks.  Allocates nothing and
+ * issues no memset, so it may be captured into a hipGraph.  A batch may be
+ * launched any number of times; launches of one batch must not overlap. */
+int bcn_batch_launch(bcn_batch* b, void* hip_stream);
+
+/* As bcn_batch_launch through a hipGraph: the first call captures the launch
+ * on a private stream and instantiates it; every call launches the graph on
+ * hip_stream. */
+int bcn_batch_launch_graph(bcn_batch* b, void* hip_stream);
+
+/* Times candidate configurations on this batch (each launched `reps` times
+ * on hip_stream, best time kept), keeps the fastest and returns it in
+ * *chosen (may be NULL).  Synchronises hip_stream.  The output buffers are
+ * rewritten with the same bytes by every run. */
+int bcn_batch_autotune(bcn_batch* b, void* hip_stream, int reps, bcn_batch_config* chosen);
+
+int bcn_batch_get_info(const bcn_batch* b, bcn_batch_info* info);
+
+/* Text of the last failure on this batch (empty if none). */
+const char* bcn_batch_last_error(const bcn_batch* b);
+
+/* Frees the work queue and graph.  Synchronise the launch stream first. */
+void bcn_batch_destroy(bcn_batch* b);
+
 #ifdef __cplusplus
 }
 #endif
