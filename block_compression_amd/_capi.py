@@ -53,7 +53,20 @@ SIGNATURES = [
     ("bcn_batch_get_info", _i32, [_vp, _vp]),
     ("bcn_batch_last_error", ctypes.c_char_p, [_vp]),
     ("bcn_batch_destroy", None, [_vp]),
+    ("bcn_decode_endpoints", _i32, [_i32, _vp, _u32, _vp, _vp]),
+    ("bcn_decode_with_endpoints", _i32, [_i32, _vp, _vp, _u32, _i32, _vp, _vp]),
+    ("bcn_decode_with_endpoints_backward", _i32, [_i32, _vp, _vp, _u32, _vp, _vp]),
+    ("bcn_bc7_refine", _i32, [_vp, _vp, _u32, _u32, _vp, _u32, _vp, _vp, _vp, _vp, _sz, _vp]),
+    ("bcn_bc7_refine_workspace_size", _sz, [_u32]),
 ]
+
+
+class RefineParams(ctypes.Structure):
+    """bcn_refine_params."""
+
+    _fields_ = [("steps", _u32), ("lr", ctypes.c_float), ("beta1", ctypes.c_float),
+                ("beta2", ctypes.c_float), ("eps", ctypes.c_float), ("channels", _u32),
+                ("reassign", _u32), ("pbits", _u32)]
 
 
 class BatchConfig(ctypes.Structure):

@@ -101,6 +101,18 @@ BCN_HD uint32_t fresh(uint32_t x) {
   return x;
 }
 
+// A pointer the compiler must treat as unknown (see fresh): loads through it
+// cannot be hoisted out of the loop that launders it.
+template <typename T>
+BCN_HD const T* fresh_ptr(const T* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm volatile("" : "+v"(p));
+#else
+  asm volatile("" : "+r"(p));
+#endif
+  return p;
+}
+
 // Makes x depend on `dep` without changing it: code reading x cannot be
 // scheduled before `dep` is computed.  Used to keep two independent fits in
 // sequence instead of interleaved (interleaving doubles the live registers).

@@ -72,6 +72,27 @@ def _value_noise(lattice: np.ndarray, freq: int, fx: np.ndarray, fy: np.ndarray)
 
 def tile_image_rgba8(width: int, height: int, seed: int = 1, noise_frac: float = 0.0) -> np.ndarray:
     """(height, width, 4) uint8 image; width and height multiples of 8."""
+    rgb = np.clip(np.floor(_tile_rgb(width, height, seed, noise_frac) + 0.5), 0, 255).astype(np.uint8)
+    img = np.empty((height, width, 4), dtype=np.uint8)
+    img[..., :3] = rgb
+    img[..., 3] = 255
+    return img
+
+
+def tile_target_rgba_f32(width: int, height: int, seed: int = 1, noise_frac: float = 0.0) -> np.ndarray:
+    """(height, width, 4) float32 RGBA target in [0, 1]: the tile generator's
+    values before rounding to 8 bits, divided by 255 and clipped; alpha is the
+    red channel of a second generator image (seed + 7919), likewise."""
+    rgb = _tile_rgb(width, height, seed, noise_frac)
+    a = _tile_rgb(width, height, seed + 7919, noise_frac)[..., 0]
+    out = np.empty((height, width, 4), dtype=np.float32)
+    out[..., :3] = np.clip(rgb / 255.0, 0.0, 1.0)
+    out[..., 3] = np.clip(a / 255.0, 0.0, 1.0)
+    return out
+
+
+def _tile_rgb(width: int, height: int, seed: int, noise_frac: float) -> np.ndarray:
+    """(height, width, 3) float64 tile-generator values (0..255 scale, before rounding)."""
     if width % 8 or height % 8:
         raise ValueError("width and height must be multiples of 8")
     tw, th = width // 8, height // 8
@@ -134,11 +155,7 @@ def tile_image_rgba8(width: int, height: int, seed: int = 1, noise_frac: float =
     if sel.size:
         out[sel] = _draws(seed, sel, 5, 192).reshape(-1, 8, 8, 3) * 256.0
 
-    rgb = np.clip(np.floor(out + 0.5), 0, 255).astype(np.uint8)
-    img = np.empty((th, 8, tw, 8, 4), dtype=np.uint8)
-    img[..., :3] = rgb.reshape(th, tw, 8, 8, 3).transpose(0, 2, 1, 3, 4)
-    img[..., 3] = 255
-    return img.reshape(height, width, 4)
+    return out.reshape(th, tw, 8, 8, 3).transpose(0, 2, 1, 3, 4).reshape(height, width, 3)
 
 
 def hdr_image_rgba16f(width: int, height: int, seed: int = 3) -> np.ndarray:

@@ -205,6 +205,62 @@ const char* bcn_batch_last_error(const bcn_batch* b);
 /* Frees the work queue and graph.  Synchronise the launch stream first. */
 void bcn_batch_destroy(bcn_batch* b);
 
+/* ---- Decode adjoints and gradient endpoint refinement ----------------------------
+ *
+ * No counterpart in the reference.  The decodes of BC7 (decode_block_bc7,
+ * src/decode/block.rs:801-1252) and BC1 (decode_color_block, :124-197) as
+ * functions of continuous endpoints, with everything else in the block held
+ * fixed: mode, partition, rotation, p-bits and indices (BC1: the colour mode
+ * and indices).  Parameters per block, float, in the mode's quantised units
+ * without p-bits: BC7 x[6][4] (endpoint e = 2*subset + {0,1}, channel RGBA;
+ * unused entries 0), BC1 x[2][3] (endpoint, RGB in 5/6/5-bit units).
+ * Texels: float[16][4] per block, row-major texels, 0..255 units.
+ * Forward, smooth = 0: straight-through, the exact decode of clamp(rint(x));
+ * smooth = 1: the decode's linear relaxation.  Backward: the transpose of the
+ * relaxation's Jacobian (the straight-through gradient).  All device
+ * pointers; blocks packed (16 or 8 bytes each). */
+int bcn_decode_endpoints(bcn_variant variant, const void* blocks, uint32_t nblocks, float* x,
+                         void* hip_stream);
+int bcn_decode_with_endpoints(bcn_variant variant, const void* blocks, const float* x,
+                              uint32_t nblocks, int smooth, float* texels, void* hip_stream);
+int bcn_decode_with_endpoints_backward(bcn_variant variant, const void* blocks,
+                                       const float* grad_texels, uint32_t nblocks, float* grad_x,
+                                       void* hip_stream);
+
+typedef struct bcn_refine_params {
+  uint32_t steps;      /* Adam steps                                           */
+  float lr;            /* step size, in quantised endpoint units               */
+  float beta1, beta2;  /* Adam moments                                          */
+  float eps;
+  uint32_t channels;   /* 3: minimise RGB squared error, 4: RGBA              */
+  uint32_t reassign;   /* 0: indices fixed; k > 0: after every k-th step, give
+                          each texel its best index for the current endpoints
+                          (mode, partition, rotation stay fixed)            */
+  uint32_t pbits;      /* with reassign: also try each p-bit flipped, keep
+                          flips that lower the error                         */
+} bcn_refine_params;
+
+/* Gradient endpoint refinement of a BC7 block image against a float RGBA
+ * target (width x height texels, 0..1 per channel, target_pitch bytes per row,
+ * 16-byte aligned): per block, Adam on the endpoints from the block's own,
+ * straight-through rounding, the adjoint above for the gradient of the
+ * squared error, optionally alternating with index (and p-bit) reassignment;
+ * each iterate is scored by its exact decode and the best (the input
+ * included) is written, with anchor indices made encodable by the usual
+ * endpoint swap, so no block's error grows.  blocks_out may equal
+ * blocks_in.  sse_before / sse_after (may be NULL): per-block squared error in
+ * 0..255 units.  workspace: device memory of bcn_bc7_refine_workspace_size
+ * bytes; calls sharing a workspace must not overlap.  Replaces, as an option, the least-squares refinement of the
+ * encoder (refine_iterations) as a pass after it. */
+int bcn_bc7_refine(const void* blocks_in, void* blocks_out, uint32_t width, uint32_t height,
+                   const float* target, uint32_t target_pitch, const bcn_refine_params* params,
+                   float* sse_before, float* sse_after, void* workspace, size_t workspace_size,
+                   void* hip_stream);
+
+/* Bytes of device workspace bcn_bc7_refine needs for nblocks blocks (a sort
+ * of the blocks by mode, so that each wave refines blocks of one mode). */
+size_t bcn_bc7_refine_workspace_size(uint32_t nblocks);
+
 #ifdef __cplusplus
 }
 #endif

@@ -126,3 +126,90 @@ extern "C" int kcheck_decode(int fmt, int is_signed, const uint8_t* blocks, size
   }
   return 0;
 }
+
+// ---- decode adjoints and refinement (decode_adjoint.hpp) ----------------------
+#include "../../block_compression_amd/csrc/decode_adjoint.hpp"
+
+extern "C" void kcheck_bc7_repack(const uint32_t* blocks, size_t n, uint32_t* out) {
+  for (size_t b = 0; b < n; b++) {
+    bcn::Bc7Form f;
+    bcn::bc7_parse(blocks + 4 * b, f);
+    bcn::bc7_pack(f, out + 4 * b);
+  }
+}
+
+// fmt 6 = BC7 (x: 24 floats per block), 0 = BC1 (x: 6 floats per block).
+extern "C" void kcheck_endpoints(int fmt, const uint32_t* blocks, size_t n, float* x) {
+  for (size_t b = 0; b < n; b++) {
+    if (fmt == 6) {
+      bcn::Bc7Form f;
+      bcn::bc7_parse(blocks + 4 * b, f);
+      for (int e = 0; e < 6; e++)
+        for (int c = 0; c < 4; c++) x[b * 24 + e * 4 + c] = (float)f.q[e][c];
+    } else {
+      bcn::Bc1Form f;
+      bcn::bc1_parse(blocks + 2 * b, f);
+      for (int e = 0; e < 2; e++)
+        for (int c = 0; c < 3; c++) x[b * 6 + e * 3 + c] = (float)f.q[e][c];
+    }
+  }
+}
+
+extern "C" void kcheck_forward(int fmt, const uint32_t* blocks, const float* x, size_t n, int smooth,
+                               float* texels) {
+  for (size_t b = 0; b < n; b++) {
+    float t[16][4];
+    if (fmt == 6) {
+      bcn::Bc7Form f;
+      bcn::bc7_parse(blocks + 4 * b, f);
+      float xv[6][4];
+      for (int e = 0; e < 6; e++)
+        for (int c = 0; c < 4; c++) xv[e][c] = x[b * 24 + e * 4 + c];
+      bcn::bc7_forward(f, xv, smooth != 0, t);
+    } else {
+      bcn::Bc1Form f;
+      bcn::bc1_parse(blocks + 2 * b, f);
+      float xv[2][3];
+      for (int e = 0; e < 2; e++)
+        for (int c = 0; c < 3; c++) xv[e][c] = x[b * 6 + e * 3 + c];
+      bcn::bc1_forward(f, xv, smooth != 0, t);
+    }
+    memcpy(texels + b * 64, t, sizeof(t));
+  }
+}
+
+extern "C" void kcheck_backward(int fmt, const uint32_t* blocks, const float* g, size_t n, float* gx) {
+  for (size_t b = 0; b < n; b++) {
+    float gv[16][4];
+    memcpy(gv, g + b * 64, sizeof(gv));
+    if (fmt == 6) {
+      bcn::Bc7Form f;
+      bcn::bc7_parse(blocks + 4 * b, f);
+      float o[6][4];
+      bcn::bc7_backward(f, gv, o);
+      memcpy(gx + b * 24, o, sizeof(o));
+    } else {
+      bcn::Bc1Form f;
+      bcn::bc1_parse(blocks + 2 * b, f);
+      float o[2][3];
+      bcn::bc1_backward(f, gv, o);
+      memcpy(gx + b * 6, o, sizeof(o));
+    }
+  }
+}
+
+// target: float[n][16][4] in 0..255 units (block-major).
+extern "C" void kcheck_bc7_refine(const uint32_t* blocks, const float* target, size_t n, uint32_t steps,
+                                  float lr, uint32_t channels, uint32_t reassign, uint32_t* out, float* sse0,
+                                  float* sse1) {
+  bcn::RefineParams rp;
+  rp.steps = steps;
+  rp.lr = lr;
+  rp.beta1 = 0.9f;
+  rp.beta2 = 0.999f;
+  rp.eps = 1e-8f;
+  rp.channels = channels;
+  rp.reassign = reassign & 0xFFFFu;
+  rp.pbits = reassign >> 16;
+  for (size_t b = 0; b < n; b++) sse1[b] = bcn::bc7_refine_block(blocks + 4 * b, target + b * 64, 1, rp, out + 4 * b, sse0 + b);
+}

@@ -57,3 +57,53 @@ def decode(fmt: int, blocks: np.ndarray, is_signed: bool = False) -> np.ndarray:
     out = np.zeros(n * 48, np.uint16) if fmt == 5 else np.zeros(n * 16, np.uint32)
     lib().kcheck_decode(fmt, int(is_signed), _p(blocks), n, _p(out))
     return out
+
+
+def bc7_repack(blocks: np.ndarray) -> np.ndarray:
+    blocks = np.ascontiguousarray(blocks, dtype=np.uint8)
+    out = np.zeros_like(blocks)
+    lib().kcheck_bc7_repack(_p(blocks), blocks.size // 16, _p(out))
+    return out
+
+
+def _nparams(fmt):
+    return 24 if fmt == 6 else 6
+
+
+def endpoints(fmt: int, blocks: np.ndarray) -> np.ndarray:
+    blocks = np.ascontiguousarray(blocks, dtype=np.uint8)
+    n = blocks.size // (16 if fmt == 6 else 8)
+    x = np.zeros(n * _nparams(fmt), np.float32)
+    lib().kcheck_endpoints(fmt, _p(blocks), n, _p(x))
+    return x.reshape(n, -1)
+
+
+def forward(fmt: int, blocks: np.ndarray, x: np.ndarray, smooth: bool) -> np.ndarray:
+    blocks = np.ascontiguousarray(blocks, dtype=np.uint8)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    n = x.shape[0]
+    t = np.zeros((n, 16, 4), np.float32)
+    lib().kcheck_forward(fmt, _p(blocks), _p(x), n, int(smooth), _p(t))
+    return t
+
+
+def backward(fmt: int, blocks: np.ndarray, g: np.ndarray) -> np.ndarray:
+    blocks = np.ascontiguousarray(blocks, dtype=np.uint8)
+    g = np.ascontiguousarray(g, dtype=np.float32)
+    n = g.shape[0]
+    gx = np.zeros((n, _nparams(fmt)), np.float32)
+    lib().kcheck_backward(fmt, _p(blocks), _p(g), n, _p(gx))
+    return gx
+
+
+def bc7_refine(blocks: np.ndarray, target: np.ndarray, steps: int, lr: float, channels: int,
+               reassign: int = 0):
+    blocks = np.ascontiguousarray(blocks, dtype=np.uint8)
+    target = np.ascontiguousarray(target, dtype=np.float32)
+    n = target.shape[0]
+    out = np.zeros_like(blocks)
+    s0 = np.zeros(n, np.float32)
+    s1 = np.zeros(n, np.float32)
+    lib().kcheck_bc7_refine(_p(blocks), _p(target), n, steps, ctypes.c_float(lr), channels, reassign,
+                            _p(out), _p(s0), _p(s1))
+    return out, s0, s1
