@@ -18,6 +18,7 @@ seeded sample of blocks -- byte identity and the reference harness's PSNR
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import math
 import os
@@ -53,6 +54,89 @@ def psnr_rgb(orig, dec) -> float:
     d = lut[orig[..., :3]] - lut[dec[..., :3]]
     mse = (d * d).reshape(-1, 3).mean(axis=0).mean()
     return 0.0 if mse == 0.0 else 20.0 * math.log10(1.0 / math.sqrt(mse))
+
+
+def bench_formats(img, reps: int = 5, launches: int = 20) -> dict:
+    """The other kernels on their configurations: BC1/BC3/BC4/BC5 encode of the
+    same 4096x4096 texture, BC6H 'slow' encode of the 2048x2048 HDR texture,
+    and the GPU decode of the 4096x4096 BC7 output.  Each figure is the best of
+    `reps` runs of `launches` back-to-back launches between two HIP events,
+    divided by `launches` (one launch of BC4 is ~10 us, comparable to launch
+    gaps).  HBM roofline against 8 TB/s with the algorithmic bytes per block
+    (64 B read + block out for RGBA8 sources, 128 B + 16 B for BC6H, 16 B +
+    64 B for decode)."""
+    import numpy as np
+    import torch
+
+    from block_compression_amd import GpuBlockCompressor, synth
+    from block_compression_amd._capi import check, lib
+    from block_compression_amd.settings import BC6HSettings, BC7Settings, CompressionVariant
+
+    stream = torch.cuda.current_stream()
+    comp = GpuBlockCompressor(0)
+    out = {}
+
+    def timed(fn):
+        best = None
+        for _ in range(reps):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            a.record(stream)
+            for _ in range(launches):
+                fn()
+            b.record(stream)
+            torch.cuda.synchronize()
+            ms = a.elapsed_time(b) / launches
+            best = ms if best is None else min(best, ms)
+        return best
+
+    def entry(ms, w, h, bytes_per_block, bound="hbm"):
+        blocks = (w // 4) * (h // 4)
+        gbs = blocks * bytes_per_block / (ms * 1e-3) / 1e9
+        return {"ms": round(ms, 4), "mpix_s": round(w * h / (ms * 1e-3) / 1e6, 1),
+                "bytes_per_block": bytes_per_block, "achieved_gbs": round(gbs, 1),
+                "hbm_frac": round(gbs / PEAK_HBM_GBS, 4), "bound": bound}
+
+    W, H = img.shape[1], img.shape[0]
+    tex = torch.from_numpy(img).cuda()
+    for name, v in [("bc1", CompressionVariant.BC1()), ("bc3", CompressionVariant.BC3()),
+                    ("bc4", CompressionVariant.BC4()), ("bc5", CompressionVariant.BC5())]:
+        buf = torch.empty(v.blocks_byte_size(W, H), dtype=torch.uint8, device="cuda")
+
+        def run(v=v, buf=buf):
+            comp.add_compression_task(v, tex, W, H, buf)
+            comp.compress()
+
+        run()
+        out[f"{name}_encode_{W}"] = entry(timed(run), W, H, 64 + v.block_byte_size())
+    # BC6H slow, 2048x2048 RGBA16F.
+    hdr = synth.hdr_image_rgba16f(2048, 2048, 3)
+    htex = torch.from_numpy(hdr.view(np.uint16)).cuda().view(torch.float16)
+    v6 = CompressionVariant.BC6H(BC6HSettings.slow())
+    b6 = torch.empty(v6.blocks_byte_size(2048, 2048), dtype=torch.uint8, device="cuda")
+
+    def run6():
+        comp.add_compression_task(v6, htex, 2048, 2048, b6)
+        comp.compress()
+
+    run6()
+    e6 = entry(timed(run6), 2048, 2048, 144, bound="valu")
+    out["bc6h_slow_encode_2048"] = e6
+    # Decode of the BC7 opaque_basic blocks of the 4096^2 texture.
+    v7 = CompressionVariant.BC7(BC7Settings.opaque_basic())
+    b7 = torch.empty(v7.blocks_byte_size(W, H), dtype=torch.uint8, device="cuda")
+    comp.add_compression_task(v7, tex, W, H, b7)
+    comp.compress()
+    rgba = torch.empty((H, W, 4), dtype=torch.uint8, device="cuda")
+
+    def rund():
+        check(lib().bcn_decode_rgba8(6, b7.data_ptr(), b7.numel(), W, H, rgba.data_ptr(), W * 4,
+                                     ctypes.c_void_p(stream.cuda_stream)))
+
+    rund()
+    out[f"bc7_decode_{W}"] = entry(timed(rund), W, H, 80)
+    torch.cuda.synchronize()
+    return out
 
 
 def bench_batched(n_textures: int, reps: int, oracle_levels: int, threads: int) -> dict:
@@ -282,6 +366,7 @@ def main() -> int:
     ap.add_argument("--grad-size", type=int, default=1024,
                     help="side of the gradient-refinement target (0 = skip it)")
     ap.add_argument("--grad-preset", default="alpha_slow")
+    ap.add_argument("--no-formats", action="store_true", help="skip the other-format kernels")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times "
                          "(reference kernel for scaling HBM byte counters) and exit")
@@ -425,6 +510,11 @@ def main() -> int:
             "hbm_frac": round(nblocks * 80 / (kernel_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 5),
         }
 
+    formats = None
+    if not args.no_formats:
+        log("bench: other formats (BC1/BC3/BC4/BC5 4096^2, BC6H slow 2048^2, BC7 decode)")
+        formats = bench_formats(img)
+
     batched = None
     if args.batched_textures > 0:
         batched = bench_batched(args.batched_textures, args.batched_reps, 24, threads)
@@ -461,6 +551,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu_baseline,
         "parity": parity,
+        "other_kernels": formats,
         "batched": batched,
         "grad_refined": grad_refined,
     }
