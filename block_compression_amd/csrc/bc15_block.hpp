@@ -41,7 +41,7 @@ BCN_HD uint32_t bc1_fast_quant(const uint32_t px[16], int p0, int p1) {
   for (int p = 0; p < 3; p++) dir[p] = c1[p] - c0[p];
   // Integer-valued, exact in any order.
   float sq_norm = dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2];
-  float rsq_norm = 1.0f / sq_norm;
+  float rsq_norm = rcp_cr(sq_norm);
   float t = rsq_norm * 3.0f;
 #pragma unroll
   for (int p = 0; p < 3; p++) dir[p] = dir[p] * t;
@@ -104,7 +104,7 @@ BCN_HD void bc1_core(const uint32_t px[16], uint32_t out[2]) {
     a2 = x2;
     if (i % 2 == 1) {
       float norm_sq = x0 * x0 + x1 * x1 + x2 * x2;
-      float rnorm = 1.0f / sqrtf(norm_sq);
+      float rnorm = rcp_cr(sqrt_cr(norm_sq));
       a0 *= rnorm;
       a1 *= rnorm;
       a2 *= rnorm;
@@ -126,7 +126,7 @@ BCN_HD void bc1_core(const uint32_t px[16], uint32_t out[2]) {
     max_dot += 0.5f;
   }
   float norm_sq = a0 * a0 + a1 * a1 + a2 * a2;
-  float rnorm_sq = 1.0f / norm_sq;
+  float rnorm_sq = rcp_cr(norm_sq);
   float axis[3] = {a0, a1, a2};
   float e0[3], e1[3];
 #pragma unroll
@@ -164,7 +164,7 @@ BCN_HD void bc1_core(const uint32_t px[16], uint32_t out[2]) {
       float cxx = (float)(144 - 6 * sum_q + sum_qq);
       float cyy = (float)sum_qq;
       float cxy = (float)(3 * sum_q - sum_qq);
-      float scale = 3.0f * (1.0f / (cxx * cyy - cxy * cxy));
+      float scale = 3.0f * rcp_cr(cxx * cyy - cxy * cxy);
 #pragma unroll
       for (int p = 0; p < 3; p++) {
         // sum[p] = dc[p] * 16.0 == s[p] exactly; atb2 = 3*sum - atb1 exact.

@@ -169,7 +169,7 @@ template <int CH, int ROT = 0>
 BCN_HD void pca_segment(const uint32_t px[16], uint32_t mask, float e0[4], float e1[4],
                         uint32_t sh = 0) {
   Moments m = moments<CH, ROT>(px, mask, sh);
-  float s14 = (float)m.n, rn = 1.0f / s14;
+  float s14 = (float)m.n, rn = rcp_normal(s14);  // n in [1, 16]
   float dc[4];
 #pragma unroll
   for (int p = 0; p < 4; p++) dc[p] = p < CH ? div_rcp((float)m.s[p], s14, rn) : 0.0f;
@@ -189,6 +189,11 @@ BCN_HD void pca_segment(const uint32_t px[16], uint32_t mask, float e0[4], float
   float ext0 = INFINITY, ext1 = -INFINITY;
 #pragma unroll
   for (int k = 0; k < 16; k++) {
+    // Only texels in the mask are projected.  Where the mask is the same for
+    // the whole wave (the partition loops of modes 0 and 2 run partition p on
+    // every lane) this is a scalar branch that skips the texel; otherwise it
+    // is an execution mask over the same instructions.
+    if (!((mask >> k) & 1u)) continue;
     // The reference's running sum starts at 0.0; dropping that leading add
     // can only turn a +0.0 into -0.0, and the sign of a zero projection never
     // reaches an output (ext*axis + dc rounds it away).
@@ -197,9 +202,8 @@ BCN_HD void pca_segment(const uint32_t px[16], uint32_t mask, float e0[4], float
     dot = dot + axis[1] * (chf(t, 1) - dc[1]);
     dot = dot + axis[2] * (chf(t, 2) - dc[2]);
     if (CH == 4) dot = dot + axis[3] * (chf(t, 3) - dc[3]);
-    bool on = (mask >> k) & 1u;
-    ext0 = on ? fminf(ext0, dot) : ext0;
-    ext1 = on ? fmaxf(ext1, dot) : ext1;
+    ext0 = fminf(ext0, dot);
+    ext1 = fmaxf(ext1, dot);
   }
   if (ext1 - ext0 < 1.0f) {
     ext0 -= 0.5f;
@@ -354,7 +358,9 @@ BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pat
     }
     fdiv[j] = (float)div;
     // div == 0 gives +inf here and NaN below (proj is then 0), as 0 / 0 does.
-    frcp[j] = 1.0f / fdiv[j];
+    // div in [1, 260100], or 0: then rcp_normal gives NaN instead of +inf,
+    // and proj is 0, so the quotient below is NaN either way.
+    frcp[j] = rcp_normal(fdiv[j]);
   }
   int total = 0;
   qblock[0] = 0;
@@ -431,7 +437,7 @@ BCN_HD void opt_endpoints(const uint32_t px[16], const uint32_t qblock[2], uint3
   int det = mul24i(cxx, cyy) - mul24i(cxy, cxy);
   float scale = (float)L1 / (float)det;
   float fcxx = (float)cxx, fcyy = (float)cyy, fcxy = (float)cxy;
-  float fcnt = (float)cnt, rcnt = 1.0f / fcnt;
+  float fcnt = (float)cnt, rcnt = rcp_normal(fcnt);  // cnt in [1, 16]
 #pragma unroll
   for (int p = 0; p < CH; p++) {
     float a1 = (float)atb1[p];
@@ -462,7 +468,8 @@ BCN_HD int chan(const uint32_t px[16], int k, uint32_t sh) {
 template <int BITS>
 BCN_HD int channel_quant(const uint32_t px[16], uint32_t sh, int e0, int e1, uint32_t qblock[2]) {
   constexpr int L = 1 << BITS;
-  float den = (float)(e1 - e0) + 0.001f, rden = 1.0f / den;
+  // |den| in [0.001, 255.001]
+  float den = (float)(e1 - e0) + 0.001f, rden = rcp_normal(den);
   qblock[0] = 0;
   qblock[1] = 0;
   int total = 0;
@@ -692,7 +699,7 @@ BCN_HD void partition_bounds(const uint32_t px[16], uint16_t* keys, int stride) 
         for (int i = 0; i < 10; i++) q[i] = full.q[i] - m.q[i];
       }
       float c[10];
-      covar_from_moments<CH>(c, sm, q, n, 1.0f / (float)n);
+      covar_from_moments<CH>(c, sm, q, n, rcp_normal((float)n));  // n in [1, 15]
       // get_pca_bound (common.rs:208-245).
       const float inv_var = 1.0f / (256.0f * 256.0f);
       const float eps = 0.001f * 0.001f;

@@ -85,6 +85,58 @@ BCN_HD float div_rcp(float a, float b, float y) {
   return __builtin_fmaf(r, y, q);
 }
 
+// sqrtf(x), correctly rounded, for every x.  The hardware square root
+// followed by the test of the candidates one ulp below and above with an
+// exact FMA residual (the correction the compiler's sequence applies, without
+// its rescaling of tiny inputs and its class fix-ups) equals sqrtf for x == 0
+// and every x >= 2^-104, as an exhaustive sweep of all non-negative floats on
+// the GPU shows (tests/test_gpu_fastmath.py); other inputs (tiny, negative,
+// infinite, NaN) take the compiler's sequence, a branch no wave takes in
+// practice.
+BCN_HD float sqrt_cr(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if (!(x == 0.0f || (x >= 0x1p-100f && x <= 0x1.fffffep127f))) return sqrtf(x);
+  float s = __builtin_amdgcn_sqrtf(x);
+  const uint32_t b = __float_as_uint(s);
+  const float sd = __uint_as_float(b - 1u), su = __uint_as_float(b + 1u);
+  const float rd = __builtin_fmaf(-sd, s, x), ru = __builtin_fmaf(-su, s, x);
+  s = rd <= 0.0f ? sd : s;
+  return ru > 0.0f ? su : s;
+#else
+  return sqrtf(x);
+#endif
+}
+
+// 1.0f / x for normal x with |x| < 2^126, where it is correctly rounded: the
+// hardware reciprocal and one Newton step with FMA (exhaustive GPU sweep,
+// tests/test_gpu_fastmath.py).  Used without a guard where the argument is
+// shown to lie in that range (each call site says why); for x = 0 it gives
+// NaN instead of +inf.
+BCN_HD float rcp_normal(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const float y = __builtin_amdgcn_rcpf(x);
+  return __builtin_fmaf(__builtin_fmaf(-x, y, 1.0f), y, y);
+#else
+  return 1.0f / x;
+#endif
+}
+
+// 1.0f / x, correctly rounded, for every x: the hardware reciprocal and one
+// Newton step with FMA equal the correctly rounded quotient for every normal
+// x with |x| < 2^126 (exhaustive GPU sweep, tests/test_gpu_fastmath.py);
+// other inputs take the compiler's division.
+BCN_HD float rcp_cr(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const float ax = fabsf(x);
+  if (!(ax >= 0x1p-125f && ax < 0x1p125f)) return 1.0f / x;
+  const float y = __builtin_amdgcn_rcpf(x);
+  const float e = __builtin_fmaf(-x, y, 1.0f);
+  return __builtin_fmaf(e, y, y);
+#else
+  return 1.0f / x;
+#endif
+}
+
 // x / 255.0f and x / 65535.0f for every finite x (see div_rcp).
 BCN_HD float div255(float x) { return div_rcp(x, 255.0f, 1.0f / 255.0f); }
 BCN_HD float div65535(float x) { return div_rcp(x, 65535.0f, 1.0f / 65535.0f); }

@@ -1,0 +1,59 @@
+// Exhaustive GPU check of the kernels' square root and reciprocal
+// (sqrt_cr, rcp_cr in block_compression_amd/csrc/bcn_common.hpp) against the
+// compiler's correctly rounded sqrtf and 1.0f / x, built with the same flags
+// as the library.  Every float bit pattern in [lo, hi) is tested; mismatches
+// are counted per exponent field (bits 23..30).  Mode 0 tests the functions as
+// the library uses them; mode 1 their unguarded fast paths, to show where the
+// guards are needed.  Test infrastructure only: the library does not link this.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../block_compression_amd/csrc/bcn_common.hpp"
+
+__device__ float fast_sqrt_unguarded(float x) {
+  float s = __builtin_amdgcn_sqrtf(x);
+  const uint32_t b = __float_as_uint(s);
+  const float sd = __uint_as_float(b - 1u), su = __uint_as_float(b + 1u);
+  const float rd = __builtin_fmaf(-sd, s, x), ru = __builtin_fmaf(-su, s, x);
+  s = rd <= 0.0f ? sd : s;
+  return ru > 0.0f ? su : s;
+}
+
+__device__ float fast_rcp_unguarded(float x) {
+  const float y = __builtin_amdgcn_rcpf(x);
+  return __builtin_fmaf(__builtin_fmaf(-x, y, 1.0f), y, y);
+}
+
+__global__ void fastmath_kernel(uint32_t lo, uint32_t hi, int mode, unsigned long long* bad_sqrt,
+                                unsigned long long* bad_rcp) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t u = (uint64_t)lo + blockIdx.x * blockDim.x + threadIdx.x; u < hi; u += stride) {
+    const uint32_t bits = (uint32_t)u;
+    const float x = __uint_as_float(bits);
+    const uint32_t ex = (bits >> 23) & 0xFFu;
+    const float s = mode ? fast_sqrt_unguarded(x) : bcn::sqrt_cr(x);
+    const float r = mode ? fast_rcp_unguarded(x) : bcn::rcp_cr(x);
+    if (__float_as_uint(s) != __float_as_uint(sqrtf(x))) atomicAdd(&bad_sqrt[ex], 1ull);
+    // NaN results compare by class (1/x is never NaN for a non-NaN x here).
+    if (__float_as_uint(r) != __float_as_uint(1.0f / x)) atomicAdd(&bad_rcp[ex], 1ull);
+  }
+}
+
+// Counts for every exponent value (256 each) into host arrays.  Returns 0 or a
+// hipError_t.
+extern "C" int fastmath_check(uint32_t lo, uint32_t hi, int mode, unsigned long long* bad_sqrt,
+                              unsigned long long* bad_rcp) {
+  unsigned long long* d = nullptr;
+  hipError_t e = hipMalloc(&d, 512 * sizeof(unsigned long long));
+  if (e != hipSuccess) return (int)e;
+  e = hipMemset(d, 0, 512 * sizeof(unsigned long long));
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(fastmath_kernel, dim3(8192), dim3(256), 0, 0, lo, hi, mode, d, d + 256);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(bad_sqrt, d, 256 * sizeof(unsigned long long), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(bad_rcp, d + 256, 256 * sizeof(unsigned long long), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  return (int)e;
+}
