@@ -20,6 +20,12 @@ from .compressor import (  # noqa: F401
     decode_rgba8,
 )
 from ._capi import BcnError  # noqa: F401
+from . import dds, decode  # noqa: F401
+from .decode import (  # noqa: F401
+    decompress_blocks_as_rgba8,
+    decompress_blocks_as_rgba16f,
+    decompress_blocks_as_rgba32f,
+)
 
 __all__ = [
     "BC6HSettings",
@@ -33,4 +39,9 @@ __all__ = [
     "compress_rgba8",
     "decode_rgba8",
     "decode_bc6h_rgba16f",
+    "decompress_blocks_as_rgba8",
+    "decompress_blocks_as_rgba16f",
+    "decompress_blocks_as_rgba32f",
+    "dds",
+    "decode",
 ]

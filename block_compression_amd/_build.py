@@ -10,6 +10,7 @@ from __future__ import annotations
 import glob
 import os
 import subprocess
+from concurrent.futures import ThreadPoolExecutor
 import sys
 
 PKG = os.path.dirname(os.path.abspath(__file__))
@@ -39,10 +40,28 @@ def sources():
     return sorted(glob.glob(os.path.join(PKG, "csrc", "*.hip")))
 
 
-def build(verbose: bool = False) -> str:
+def _compile(src: str, obj: str, verbose: bool) -> None:
+    cmd = [HIPCC, *[f for f in FLAGS if f != "-shared"], "-I", os.path.join(ROOT, "include"),
+           "-c", src, "-o", obj]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build(verbose: bool = False, jobs: int = 0) -> str:
+    """Compiles each source to an object in parallel (one hipcc per file),
+    then links the shared library."""
     os.makedirs(LIB_DIR, exist_ok=True)
+    obj_dir = os.path.join(LIB_DIR, "obj")
+    os.makedirs(obj_dir, exist_ok=True)
+    srcs = sources()
+    objs = [os.path.join(obj_dir, os.path.basename(s) + ".o") for s in srcs]
+    jobs = jobs or min(len(srcs), max(1, min(8, os.cpu_count() or 1)))
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        for f in [pool.submit(_compile, s, o, verbose) for s, o in zip(srcs, objs)]:
+            f.result()
     tmp = LIB_PATH + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-I", os.path.join(ROOT, "include"), *sources(), "-o", tmp]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -31,6 +31,7 @@ STATUS_NAMES = {
 
 SRC_RGBA8 = 0
 SRC_RGBA16F = 1
+DECODE_REFERENCE_LAYOUT = 1
 
 # (name, restype, argtypes) of every exported symbol.
 SIGNATURES = [
@@ -45,6 +46,12 @@ SIGNATURES = [
     ("bcn_blocks_byte_size", _sz, [_i32, _u32, _u32]),
     ("bcn_decode_rgba8", _i32, [_i32, _vp, _sz, _u32, _u32, _vp, _u32, _vp]),
     ("bcn_decode_bc6h_rgba16f", _i32, [_vp, _sz, _u32, _u32, _i32, _vp, _u32, _vp]),
+    ("bcn_decode_rgba16f", _i32, [_i32, _vp, _sz, _u32, _u32, _vp, _u32, _u32, _vp]),
+    ("bcn_decode_rgba32f", _i32, [_i32, _vp, _sz, _u32, _u32, _vp, _u32, _u32, _vp]),
+    ("bcn_bc6h_srgb8_thresholds", None, [_vp]),
+    ("bcn_dxgi_format", _u32, [_i32]),
+    ("bcn_dds_data_size", _sz, [_u32, _u32, _u32, _u32, _u32]),
+    ("bcn_dds_write_header", _i32, [_u32, _u32, _u32, _u32, _u32, _u32, _vp, _sz]),
     ("bcn_partition_tables", None, [_vp, _vp, _vp, _vp]),
     ("bcn_batch_create", _i32, [_vp, _vp, ctypes.POINTER(_vp)]),
     ("bcn_batch_launch", _i32, [_vp, _vp]),

@@ -1,17 +1,28 @@
 // Block decode kernels: one lane per 4x4 block.
 //
 // The reference decodes on the CPU only (src/decode.rs); these are the GPU
-// counterparts of decompress_blocks_as_rgba8 (src/decode.rs:336-387) and of
-// decode_block_bc6h into RGBA16F.  Each lane reads its 8/16-byte block and
-// writes four 16-byte (RGBA8) or 32-byte (RGBA16F) row segments; adjacent
-// lanes write adjacent segments of the same rows.  HBM-bound: 80 bytes of
-// traffic per BC7 block (16 in, 64 out).
+// counterparts of its image-level drivers decompress_blocks_as_rgba8 /
+// _rgba16f / _rgba32f (src/decode.rs:336-468) and of decode_block_bc6h with
+// the signed variant.  Each lane reads its 8/16-byte block and writes four
+// row segments of 16 bytes (RGBA8), 32 bytes (RGBA16F) or 64 bytes (RGBA32F);
+// adjacent lanes write adjacent segments of the same rows.  HBM-bound: 80
+// bytes of traffic per BC7 block (16 in, 64 out).
+#include <cmath>
+
 #include "decode_block.hpp"
 #include "bcn_internal.hpp"
 
 namespace bcn {
 
 constexpr int kDecGroup = 256;
+
+// Output forms of the BC6H kernel.
+enum Bc6hOut : int {
+  kOutF16Rgba = 0,    // RGBA16F, alpha 1.0
+  kOutF32Rgba = 1,    // RGBA32F, alpha 1.0
+  kOutF16RefRgb = 2,  // the reference's RGB-at-stride-3 layout in an RGBA16F image
+  kOutF32RefRgb = 3   // the same in an RGBA32F image
+};
 
 template <int FMT>
 __global__ __launch_bounds__(kDecGroup) void decode_rgba8_kernel(const uint8_t* __restrict__ blocks,
@@ -65,6 +76,36 @@ __global__ __launch_bounds__(kDecGroup) void decode_rgba8_kernel(const uint8_t* 
         make_uint4(px[y * 4 + 0], px[y * 4 + 1], px[y * 4 + 2], px[y * 4 + 3]);
 }
 
+// BC6H -> RGBA8 (decode.rs:161-181): unsigned decode, linear_to_srgb per
+// channel, alpha 0.
+__global__ __launch_bounds__(kDecGroup) void decode_bc6h_srgb8_kernel(
+    const uint8_t* __restrict__ blocks, uint32_t bw, uint32_t nblocks, uint8_t* __restrict__ rgba,
+    uint32_t pitch, Srgb8Table tab) {
+  __shared__ uint16_t lt[256];
+  lt[threadIdx.x] = tab.t[threadIdx.x];
+  __syncthreads();
+  uint32_t idx = blockIdx.x * kDecGroup + threadIdx.x;
+  if (idx >= nblocks) return;
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(blocks + (size_t)idx * 16);
+  uint32_t in[4] = {src[0], src[1], src[2], src[3]};
+  uint16_t h[48];
+  decode_bc6h(in, false, h);
+  uint32_t yy = idx / bw, xx = idx - yy * bw;
+  uint8_t* o = rgba + (size_t)yy * 4 * pitch + (size_t)xx * 16;
+#pragma unroll
+  for (int y = 0; y < 4; y++) {
+    uint32_t px[4];
+#pragma unroll
+    for (int x = 0; x < 4; x++) {
+      int k = y * 4 + x;
+      px[x] = srgb8_code(h[k * 3], lt) | (srgb8_code(h[k * 3 + 1], lt) << 8) |
+              (srgb8_code(h[k * 3 + 2], lt) << 16);
+    }
+    *reinterpret_cast<uint4*>(o + (size_t)y * pitch) = make_uint4(px[0], px[1], px[2], px[3]);
+  }
+}
+
+template <int OUT>
 __global__ __launch_bounds__(kDecGroup) void decode_bc6h_kernel(const uint8_t* __restrict__ blocks,
                                                                 uint32_t bw, uint32_t nblocks,
                                                                 int is_signed,
@@ -76,21 +117,114 @@ __global__ __launch_bounds__(kDecGroup) void decode_bc6h_kernel(const uint8_t* _
   uint32_t in[4] = {src[0], src[1], src[2], src[3]};
   uint16_t h[48];
   decode_bc6h(in, is_signed != 0, h);
+  constexpr bool F32 = OUT == kOutF32Rgba || OUT == kOutF32RefRgb;
   uint32_t yy = idx / bw, xx = idx - yy * bw;
-  uint8_t* o = rgba + (size_t)yy * 4 * pitch + (size_t)xx * 32;
+  uint8_t* o = rgba + (size_t)yy * 4 * pitch + (size_t)xx * (F32 ? 64 : 32);
 #pragma unroll
   for (int y = 0; y < 4; y++) {
-    uint4* row = reinterpret_cast<uint4*>(o + (size_t)y * pitch);
+    uint8_t* r = o + (size_t)y * pitch;
+    const uint16_t* hr = h + y * 12;  // the row's 4 texels, RGB interleaved
+    if (OUT == kOutF16Rgba) {
+      uint4* row = reinterpret_cast<uint4*>(r);
 #pragma unroll
-    for (int x = 0; x < 4; x += 2) {
-      int k0 = y * 4 + x, k1 = k0 + 1;
-      // RGBA16F texels, alpha 1.0 (0x3C00).
-      row[x / 2] = make_uint4((uint32_t)h[k0 * 3] | ((uint32_t)h[k0 * 3 + 1] << 16),
-                              (uint32_t)h[k0 * 3 + 2] | (0x3C00u << 16),
-                              (uint32_t)h[k1 * 3] | ((uint32_t)h[k1 * 3 + 1] << 16),
-                              (uint32_t)h[k1 * 3 + 2] | (0x3C00u << 16));
+      for (int x = 0; x < 4; x += 2)
+        row[x / 2] = make_uint4((uint32_t)hr[x * 3] | ((uint32_t)hr[x * 3 + 1] << 16),
+                                (uint32_t)hr[x * 3 + 2] | (0x3C00u << 16),
+                                (uint32_t)hr[x * 3 + 3] | ((uint32_t)hr[x * 3 + 4] << 16),
+                                (uint32_t)hr[x * 3 + 5] | (0x3C00u << 16));
+    } else if (OUT == kOutF32Rgba) {
+      uint4* row = reinterpret_cast<uint4*>(r);
+#pragma unroll
+      for (int x = 0; x < 4; x++)
+        row[x] = make_uint4(__float_as_uint(half_to_float(hr[x * 3])),
+                            __float_as_uint(half_to_float(hr[x * 3 + 1])),
+                            __float_as_uint(half_to_float(hr[x * 3 + 2])), 0x3F800000u);
+    } else if (OUT == kOutF16RefRgb) {
+      // decode_block_bc6h writes texel j at j*3 (block.rs:753) from the
+      // block's RGBA16F offset (decode.rs:271-279): 12 halves, the other 4
+      // of the block's 16 untouched.
+      uint2* row = reinterpret_cast<uint2*>(r);
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+        row[j] = make_uint2((uint32_t)hr[4 * j] | ((uint32_t)hr[4 * j + 1] << 16),
+                            (uint32_t)hr[4 * j + 2] | ((uint32_t)hr[4 * j + 3] << 16));
+    } else {
+      // decode_block_bc6h_float, the same layout in f32 (block.rs:787-792,
+      // decode.rs:317-325).
+      uint4* row = reinterpret_cast<uint4*>(r);
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+        row[j] = make_uint4(__float_as_uint(half_to_float(hr[4 * j])),
+                            __float_as_uint(half_to_float(hr[4 * j + 1])),
+                            __float_as_uint(half_to_float(hr[4 * j + 2])),
+                            __float_as_uint(half_to_float(hr[4 * j + 3])));
     }
   }
+}
+
+// ---- host side --------------------------------------------------------------
+
+// binary16 -> binary32 of a non-negative finite half, exactly.
+float half_to_float_host(uint16_t h) {
+  int e = h >> 10, m = h & 1023;
+  return e == 0 ? std::ldexp((float)m, -24) : std::ldexp((float)(1024 + m), e - 25);
+}
+
+// linear_to_srgb (src/decode.rs:151-159) in f32 with the C library's powf,
+// the function Rust's f32::powf calls on Linux.
+#pragma clang fp contract(off)
+uint8_t linear_to_srgb8_host(float lin) {
+  float v = lin <= 0.0031308f ? lin * 12.92f : 1.055f * powf(lin, 1.0f / 2.4f) - 0.055f;
+  v = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+  return (uint8_t)roundf(v * 255.0f);
+}
+
+// Thresholds over every unsigned BC6H output (bit patterns 0..0x7BFF; the
+// unsigned finish_unquantize, block.rs:1391-1397, never exceeds 0x7BFF).
+const Srgb8Table& srgb8_table() {
+  static const Srgb8Table tab = [] {
+    Srgb8Table t;
+    t.t[0] = 0;
+    for (int k = 1; k < 256; k++) t.t[k] = 0xFFFF;
+    int k = 1;
+    for (uint32_t h = 0; h <= 0x7BFF && k < 256; h++) {
+      int code = linear_to_srgb8_host(half_to_float_host((uint16_t)h));
+      while (k <= code) t.t[k++] = (uint16_t)h;
+    }
+    return t;
+  }();
+  return tab;
+}
+
+bool launch_ok() { return hipGetLastError() == hipSuccess; }
+
+// decompress_blocks_as_rgba16f / _rgba32f (decode.rs:397-468): BC6H only.
+int decode_bc6h_image(bcn_variant variant, const void* blocks, size_t blocks_size, uint32_t width,
+                      uint32_t height, int is_signed, void* rgba, uint32_t rgba_pitch,
+                      uint32_t flags, bool f32, void* hip_stream) {
+  if (variant < BCN_BC1 || variant > BCN_BC7) return BCN_ERR_INVALID_ARGUMENT;
+  if (variant != BCN_BC6H) return BCN_ERR_UNSUPPORTED;
+  if (flags & ~(uint32_t)BCN_DECODE_REFERENCE_LAYOUT) return BCN_ERR_INVALID_ARGUMENT;
+  if (width % 4 || height % 4) return BCN_ERR_NOT_MULTIPLE_OF_4;
+  uint32_t bw = width / 4, nblocks = bw * (height / 4);
+  if (blocks_size != (size_t)nblocks * 16) return BCN_ERR_BUFFER_TOO_SMALL;
+  if ((uint64_t)rgba_pitch < (uint64_t)width * (f32 ? 16 : 8)) return BCN_ERR_INVALID_ARGUMENT;
+  if (nblocks == 0) return BCN_OK;
+  if (!blocks || !rgba) return BCN_ERR_INVALID_ARGUMENT;
+  if ((((uintptr_t)rgba | rgba_pitch) & 15u) || ((uintptr_t)blocks & 3u)) return BCN_ERR_MISALIGNED;
+  dim3 grid((nblocks + kDecGroup - 1) / kDecGroup), block(kDecGroup);
+  hipStream_t s = static_cast<hipStream_t>(hip_stream);
+  const uint8_t* b = static_cast<const uint8_t*>(blocks);
+  uint8_t* o = static_cast<uint8_t*>(rgba);
+  bool ref = flags & BCN_DECODE_REFERENCE_LAYOUT;
+  int out = f32 ? (ref ? kOutF32RefRgb : kOutF32Rgba) : (ref ? kOutF16RefRgb : kOutF16Rgba);
+  switch (out) {
+    case kOutF16Rgba: hipLaunchKernelGGL(decode_bc6h_kernel<kOutF16Rgba>, grid, block, 0, s, b, bw, nblocks, is_signed, o, rgba_pitch); break;
+    case kOutF32Rgba: hipLaunchKernelGGL(decode_bc6h_kernel<kOutF32Rgba>, grid, block, 0, s, b, bw, nblocks, is_signed, o, rgba_pitch); break;
+    case kOutF16RefRgb: hipLaunchKernelGGL(decode_bc6h_kernel<kOutF16RefRgb>, grid, block, 0, s, b, bw, nblocks, is_signed, o, rgba_pitch); break;
+    default: hipLaunchKernelGGL(decode_bc6h_kernel<kOutF32RefRgb>, grid, block, 0, s, b, bw, nblocks, is_signed, o, rgba_pitch); break;
+  }
+  return launch_ok() ? BCN_OK : BCN_ERR_HIP;
 }
 
 }  // namespace bcn
@@ -102,13 +236,12 @@ extern "C" {
 int bcn_decode_rgba8(bcn_variant variant, const void* blocks, size_t blocks_size, uint32_t width,
                      uint32_t height, void* rgba, uint32_t rgba_pitch, void* hip_stream) {
   if (variant < BCN_BC1 || variant > BCN_BC7) return BCN_ERR_INVALID_ARGUMENT;
-  if (variant == BCN_BC6H) return BCN_ERR_UNSUPPORTED;
   if (width % 4 || height % 4) return BCN_ERR_NOT_MULTIPLE_OF_4;
   size_t bb = (variant == BCN_BC1 || variant == BCN_BC4) ? 8 : 16;
   uint32_t bw = width / 4, nblocks = bw * (height / 4);
   // decode.rs:343-355: the block slice and the image must have the exact sizes.
   if (blocks_size != (size_t)nblocks * bb) return BCN_ERR_BUFFER_TOO_SMALL;
-  if (rgba_pitch < width * 4) return BCN_ERR_INVALID_ARGUMENT;
+  if ((uint64_t)rgba_pitch < (uint64_t)width * 4) return BCN_ERR_INVALID_ARGUMENT;
   if (nblocks == 0) return BCN_OK;
   if (!blocks || !rgba) return BCN_ERR_INVALID_ARGUMENT;
   if ((((uintptr_t)rgba | rgba_pitch) & 15u) || ((uintptr_t)blocks & 7u)) return BCN_ERR_MISALIGNED;
@@ -122,26 +255,36 @@ int bcn_decode_rgba8(bcn_variant variant, const void* blocks, size_t blocks_size
     case BCN_BC3: hipLaunchKernelGGL(decode_rgba8_kernel<BCN_BC3>, grid, block, 0, s, b, bw, nblocks, o, rgba_pitch); break;
     case BCN_BC4: hipLaunchKernelGGL(decode_rgba8_kernel<BCN_BC4>, grid, block, 0, s, b, bw, nblocks, o, rgba_pitch); break;
     case BCN_BC5: hipLaunchKernelGGL(decode_rgba8_kernel<BCN_BC5>, grid, block, 0, s, b, bw, nblocks, o, rgba_pitch); break;
+    case BCN_BC6H: hipLaunchKernelGGL(decode_bc6h_srgb8_kernel, grid, block, 0, s, b, bw, nblocks, o, rgba_pitch, srgb8_table()); break;
     default: hipLaunchKernelGGL(decode_rgba8_kernel<BCN_BC7>, grid, block, 0, s, b, bw, nblocks, o, rgba_pitch); break;
   }
-  return hipGetLastError() == hipSuccess ? BCN_OK : BCN_ERR_HIP;
+  return launch_ok() ? BCN_OK : BCN_ERR_HIP;
+}
+
+int bcn_decode_rgba16f(bcn_variant variant, const void* blocks, size_t blocks_size, uint32_t width,
+                       uint32_t height, void* rgba, uint32_t rgba_pitch, uint32_t flags,
+                       void* hip_stream) {
+  return decode_bc6h_image(variant, blocks, blocks_size, width, height, 0, rgba, rgba_pitch, flags,
+                           false, hip_stream);
+}
+
+int bcn_decode_rgba32f(bcn_variant variant, const void* blocks, size_t blocks_size, uint32_t width,
+                       uint32_t height, void* rgba, uint32_t rgba_pitch, uint32_t flags,
+                       void* hip_stream) {
+  return decode_bc6h_image(variant, blocks, blocks_size, width, height, 0, rgba, rgba_pitch, flags,
+                           true, hip_stream);
 }
 
 int bcn_decode_bc6h_rgba16f(const void* blocks, size_t blocks_size, uint32_t width,
                             uint32_t height, int is_signed, void* rgba, uint32_t rgba_pitch,
                             void* hip_stream) {
-  if (width % 4 || height % 4) return BCN_ERR_NOT_MULTIPLE_OF_4;
-  uint32_t bw = width / 4, nblocks = bw * (height / 4);
-  if (blocks_size != (size_t)nblocks * 16) return BCN_ERR_BUFFER_TOO_SMALL;
-  if (rgba_pitch < width * 8) return BCN_ERR_INVALID_ARGUMENT;
-  if (nblocks == 0) return BCN_OK;
-  if (!blocks || !rgba) return BCN_ERR_INVALID_ARGUMENT;
-  if ((((uintptr_t)rgba | rgba_pitch) & 15u) || ((uintptr_t)blocks & 3u)) return BCN_ERR_MISALIGNED;
-  dim3 grid((nblocks + kDecGroup - 1) / kDecGroup), block(kDecGroup);
-  hipLaunchKernelGGL(decode_bc6h_kernel, grid, block, 0, static_cast<hipStream_t>(hip_stream),
-                     static_cast<const uint8_t*>(blocks), bw, nblocks, is_signed,
-                     static_cast<uint8_t*>(rgba), rgba_pitch);
-  return hipGetLastError() == hipSuccess ? BCN_OK : BCN_ERR_HIP;
+  return decode_bc6h_image(BCN_BC6H, blocks, blocks_size, width, height, is_signed, rgba,
+                           rgba_pitch, 0, false, hip_stream);
+}
+
+void bcn_bc6h_srgb8_thresholds(uint16_t* out) {
+  const Srgb8Table& t = srgb8_table();
+  for (int k = 0; k < 256; k++) out[k] = t.t[k];
 }
 
 }  // extern "C"

@@ -305,4 +305,39 @@ BCN_HD void decode_bc6h(const uint32_t in[4], bool is_signed, uint16_t out[48]) 
   }
 }
 
+#if defined(__HIPCC__)
+// sRGB8 code thresholds of linear_to_srgb (src/decode.rs:151-159) over the
+// non-negative half floats: t[k] is the smallest half bit pattern whose code
+// is >= k (t[0] = 0).  Non-negative halves order like their bit patterns,
+// so code(h) = #{k >= 1 : h >= t[k]}.
+struct Srgb8Table {
+  uint16_t t[256];
+};
+
+__device__ __forceinline__ float half_to_float(uint16_t h) {
+  return (float)__builtin_bit_cast(_Float16, h);
+}
+
+// sRGB8 code estimate of a non-negative half from the hardware log2/exp2;
+// within one code of linear_to_srgb on every input.
+__device__ __forceinline__ int srgb8_estimate(uint16_t h) {
+  float lin = half_to_float(h);
+  float v = lin <= 0.0031308f
+                ? lin * 12.92f
+                : 1.055f * __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(lin) * (1.0f / 2.4f)) - 0.055f;
+  return (int)(fminf(fmaxf(v, 0.0f), 1.0f) * 255.0f + 0.5f);
+}
+
+// linear_to_srgb of an unsigned BC6H texel value, exactly: the estimate is
+// corrected by one comparison against each neighbouring threshold.
+__device__ __forceinline__ uint32_t srgb8_code(uint16_t h, const uint16_t* lt) {
+  int k = srgb8_estimate(h);
+  if (k > 0 && h < lt[k])
+    k--;
+  else if (k < 255 && h >= lt[k + 1])
+    k++;
+  return (uint32_t)k;
+}
+#endif
+
 }  // namespace bcn

@@ -130,17 +130,89 @@ size_t bcn_blocks_byte_size(bcn_variant variant, uint32_t width, uint32_t height
 
 /* GPU decode of a block image into RGBA8, the layout of
  * decompress_blocks_as_rgba8 (src/decode.rs:336-387): BC4 -> (R,0,0,0),
- * BC5 -> (R,G,0,0); BC6H is not accepted here (its RGBA8 path needs powf).
- * blocks: device pointer, ceil(w/4)*ceil(h/4) blocks; rgba: device pointer,
- * rgba_pitch bytes per row; width and height multiples of 4. */
+ * BC5 -> (R,G,0,0), BC6H -> (sRGB8(R), sRGB8(G), sRGB8(B), 0) with
+ * linear_to_srgb (decode.rs:151-181) reproduced exactly.
+ * blocks: device pointer, (w/4)*(h/4) blocks, exactly blocks_byte_size bytes
+ * (decode.rs:343-355); rgba: device pointer, rgba_pitch bytes per row, both
+ * 16-byte aligned; width and height multiples of 4. */
 int bcn_decode_rgba8(bcn_variant variant, const void* blocks, size_t blocks_size, uint32_t width,
                      uint32_t height, void* rgba, uint32_t rgba_pitch, void* hip_stream);
+
+/* Flag of bcn_decode_rgba16f / _rgba32f: write the reference's layout.  The
+ * reference's image-level 16F/32F drivers place each block at its RGBA
+ * offset but write its texels as packed RGB (texel j at element j*3,
+ * src/decode/block.rs:753 and :787-792, via src/decode.rs:271-279 and
+ * :317-325): per block row, elements 0..11 hold RGB and 12..15 are left
+ * untouched.  Without the flag the output is a proper RGBA image with
+ * alpha 1.0. */
+#define BCN_DECODE_REFERENCE_LAYOUT 1u
+
+/* decompress_blocks_as_rgba16f (src/decode.rs:397-430) on the GPU: BC6H
+ * (unsigned) only, any other variant is BCN_ERR_UNSUPPORTED (the reference
+ * panics).  rgba_pitch in bytes, >= width*8. */
+int bcn_decode_rgba16f(bcn_variant variant, const void* blocks, size_t blocks_size, uint32_t width,
+                       uint32_t height, void* rgba, uint32_t rgba_pitch, uint32_t flags,
+                       void* hip_stream);
+
+/* decompress_blocks_as_rgba32f (src/decode.rs:438-468): as above with f32
+ * texels (exact conversions of the half results); rgba_pitch >= width*16. */
+int bcn_decode_rgba32f(bcn_variant variant, const void* blocks, size_t blocks_size, uint32_t width,
+                       uint32_t height, void* rgba, uint32_t rgba_pitch, uint32_t flags,
+                       void* hip_stream);
 
 /* GPU decode of BC6H blocks into RGBA16F (decode_block_bc6h, src/decode/block.rs:262-769,
  * unsigned unless is_signed) with alpha written as 1.0 (0x3C00).  rgba_pitch in bytes. */
 int bcn_decode_bc6h_rgba16f(const void* blocks, size_t blocks_size, uint32_t width,
                             uint32_t height, int is_signed, void* rgba, uint32_t rgba_pitch,
                             void* hip_stream);
+
+/* The 256 thresholds the BC6H -> RGBA8 decode uses for linear_to_srgb
+ * (src/decode.rs:151-159): out[k] is the smallest half bit pattern in
+ * 0..0x7BFF whose sRGB8 code is >= k (out[0] = 0; 0xFFFF if none).  For
+ * table checks without a GPU. */
+void bcn_bc6h_srgb8_thresholds(uint16_t* out);
+
+/* ---- DDS container ------------------------------------------------------------------
+ *
+ * The reference CLI writes its blocks as a DDS file through the `ddsfile`
+ * crate 0.5 (Dds::new_dxgi + Dds::write, compressor/src/main.rs:313-353):
+ * "DDS " magic, DDS_HEADER and the DX10 extension header, then the block
+ * data in the buffer's order (mip levels of layer 0, then of layer 1, ...).
+ * Host-only functions; no GPU needed. */
+
+/* Bytes of the magic and both headers. */
+#define BCN_DDS_HEADER_SIZE 148u
+
+/* DDS_ALPHA_MODE values (miscFlags2); the CLI writes STRAIGHT. */
+typedef enum bcn_dds_alpha_mode {
+  BCN_DDS_ALPHA_UNKNOWN = 0,
+  BCN_DDS_ALPHA_STRAIGHT = 1,
+  BCN_DDS_ALPHA_PREMULTIPLIED = 2,
+  BCN_DDS_ALPHA_OPAQUE = 3,
+  BCN_DDS_ALPHA_CUSTOM = 4
+} bcn_dds_alpha_mode;
+
+/* The DXGI_FORMAT the CLI writes for a variant (dxgi_format,
+ * compressor/src/main.rs:355-365): BC1/2/3/7 -> *_UNORM_SRGB (72, 75, 78,
+ * 99), BC4 -> BC4_UNORM (80), BC5 -> BC5_UNORM (83), BC6H -> BC6H_UF16
+ * (95).  0 for an invalid variant. */
+uint32_t bcn_dxgi_format(bcn_variant variant);
+
+/* Bytes of block data a DDS file of this shape holds: the sum over mip
+ * levels of max(1, ceil(w_l/4)) * max(1, ceil(h_l/4)) * block bytes, times
+ * array_layers.  0 for a format that is not block-compressed (DXGI 70-84,
+ * 94-99) or a zero dimension. */
+size_t bcn_dds_data_size(uint32_t dxgi_format, uint32_t width, uint32_t height,
+                         uint32_t mip_levels, uint32_t array_layers);
+
+/* Writes the BCN_DDS_HEADER_SIZE bytes that precede the block data, as
+ * ddsfile's Dds::new_dxgi(width, height, depth None, format, mipmap_levels,
+ * array_layers, caps2 None, not a cubemap, Texture2D, alpha_mode) followed
+ * by Dds::write emits them.  The CLI's call is (format of the variant,
+ * mip_levels 1, array_layers 1, BCN_DDS_ALPHA_STRAIGHT). */
+int bcn_dds_write_header(uint32_t dxgi_format, uint32_t width, uint32_t height,
+                         uint32_t mip_levels, uint32_t array_layers, uint32_t alpha_mode,
+                         void* out, size_t out_size);
 
 /* Copies of the encoder's partition tables (pattern: 2 bits of subset per
  * texel; mask01: subset-0 mask | subset-1 mask << 16; anchors) for table

@@ -22,6 +22,14 @@ enum { V_BC1 = 0, V_BC2 = 1, V_BC3 = 2, V_BC4 = 3, V_BC5 = 4, V_BC6H = 5, V_BC7 
 
 size_t block_bytes(int v) { return (v == V_BC1 || v == V_BC4) ? 8 : 16; }
 
+// decode.rs:151-159, in f32 (Rust's f32::powf is the C library's powf).
+uint8_t linear_to_srgb(float linear) {
+  float v = linear <= 0.0031308f ? linear * 12.92f : 1.055f * std::pow(linear, 1.0f / 2.4f) - 0.055f;
+  if (v < 0.0f) v = 0.0f;
+  if (v > 1.0f) v = 1.0f;
+  return (uint8_t)std::round(v * 255.0f);
+}
+
 }  // namespace
 
 extern "C" {
@@ -200,14 +208,7 @@ int oracle_decompress_rgba8(int variant, uint32_t width, uint32_t height, const 
           for (int y = 0; y < 4; y++)
             for (int x = 0; x < 4; x++) {
               uint8_t* p = o + y * pitch + x * 4;
-              for (int c = 0; c < 3; c++) {
-                float lin = buf[y * 12 + x * 3 + c];
-                float v = lin <= 0.0031308f ? lin * 12.92f
-                                            : 1.055f * std::pow(lin, 1.0f / 2.4f) - 0.055f;
-                if (v < 0.0f) v = 0.0f;
-                if (v > 1.0f) v = 1.0f;
-                p[c] = (uint8_t)std::round(v * 255.0f);
-              }
+              for (int c = 0; c < 3; c++) p[c] = linear_to_srgb(buf[y * 12 + x * 3 + c]);
               p[3] = 0;
             }
           break;
@@ -217,6 +218,41 @@ int oracle_decompress_rgba8(int variant, uint32_t width, uint32_t height, const 
     }
   return 0;
 }
+
+// decompress_rgba16f / decompress_rgba32f + decompress_blocks_as_rgba16f /
+// _rgba32f (decode.rs:247-334, :397-468).  BC6H only (-3 otherwise, where
+// the reference panics).  The output image has width*4 elements per row and
+// each block goes to its RGBA offset, but decode_block_bc6h(_float) writes
+// texel j at j*3 (block.rs:753, :787-792): per block row, elements 0..11
+// receive RGB and 12..15 keep what the caller's buffer held.
+int oracle_decompress_rgba16f(int variant, uint32_t width, uint32_t height, const uint8_t* blocks,
+                              size_t blocks_len, uint16_t* rgba, size_t rgba_len) {
+  if (variant != V_BC6H) return -3;
+  size_t bw = (width + 3) / 4, bh = (height + 3) / 4;
+  if (blocks_len != bw * bh * 16) return -2;
+  if (rgba_len != (size_t)width * height * 4) return -4;
+  size_t pitch = (size_t)width * 4;
+  for (size_t by = 0; by < bh; by++)
+    for (size_t bx = 0; bx < bw; bx++)
+      decode_block_bc6h(blocks + (by * bw + bx) * 16, rgba + by * 4 * pitch + bx * 16, pitch, false);
+  return 0;
+}
+
+int oracle_decompress_rgba32f(int variant, uint32_t width, uint32_t height, const uint8_t* blocks,
+                              size_t blocks_len, float* rgba, size_t rgba_len) {
+  if (variant != V_BC6H) return -3;
+  size_t bw = (width + 3) / 4, bh = (height + 3) / 4;
+  if (blocks_len != bw * bh * 16) return -2;
+  if (rgba_len != (size_t)width * height * 4) return -4;
+  size_t pitch = (size_t)width * 4;
+  for (size_t by = 0; by < bh; by++)
+    for (size_t bx = 0; bx < bw; bx++)
+      decode_block_bc6h_float(blocks + (by * bw + bx) * 16, rgba + by * 4 * pitch + bx * 16, pitch,
+                              false);
+  return 0;
+}
+
+uint8_t oracle_linear_to_srgb(float linear) { return linear_to_srgb(linear); }
 
 // Primitives exposed for the brute-force and closed-form pin tests.
 void oracle_partial_sort_list(int32_t* list, size_t length, uint32_t partial_count) {

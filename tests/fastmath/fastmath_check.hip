@@ -4,11 +4,15 @@
 // as the library.  Every float bit pattern in [lo, hi) is tested; mismatches
 // are counted per exponent field (bits 23..30).  Mode 0 tests the functions as
 // the library uses them; mode 1 their unguarded fast paths, to show where the
-// guards are needed.  Test infrastructure only: the library does not link this.
+// guards are needed.  The second check sweeps the BC6H -> RGBA8 decode's sRGB8
+// code (srgb8_estimate / srgb8_code in decode_block.hpp) over every
+// non-negative finite half.  Test infrastructure only: the library does not
+// link this.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "../../block_compression_amd/csrc/bcn_common.hpp"
+#include "../../block_compression_amd/csrc/decode_block.hpp"
 
 __device__ float fast_sqrt_unguarded(float x) {
   float s = __builtin_amdgcn_sqrtf(x);
@@ -54,6 +58,37 @@ extern "C" int fastmath_check(uint32_t lo, uint32_t hi, int mode, unsigned long 
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e == hipSuccess) e = hipMemcpy(bad_sqrt, d, 256 * sizeof(unsigned long long), hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(bad_rcp, d + 256, 256 * sizeof(unsigned long long), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  return (int)e;
+}
+
+// ---- sRGB8 codes of the BC6H -> RGBA8 decode ---------------------------------
+// For every half bit pattern h in [0, n): est[h] = srgb8_estimate(h) and
+// code[h] = srgb8_code(h, table) (block_compression_amd/csrc/decode_block.hpp),
+// with `table` the library's thresholds (bcn_bc6h_srgb8_thresholds).
+
+__global__ void srgb8_kernel(uint32_t n, bcn::Srgb8Table tab, uint8_t* est, uint8_t* code) {
+  __shared__ uint16_t lt[256];
+  lt[threadIdx.x] = tab.t[threadIdx.x];
+  __syncthreads();
+  const uint32_t h = blockIdx.x * 256 + threadIdx.x;
+  if (h >= n) return;
+  est[h] = (uint8_t)bcn::srgb8_estimate((uint16_t)h);
+  code[h] = (uint8_t)bcn::srgb8_code((uint16_t)h, lt);
+}
+
+extern "C" int srgb8_check(const uint16_t* table, uint32_t n, uint8_t* est, uint8_t* code) {
+  if (n > 0x7C00u) return -1;
+  bcn::Srgb8Table tab;
+  for (int k = 0; k < 256; k++) tab.t[k] = table[k];
+  uint8_t* d = nullptr;
+  hipError_t e = hipMalloc(&d, 2 * 0x7C00);
+  if (e != hipSuccess) return (int)e;
+  hipLaunchKernelGGL(srgb8_kernel, dim3((n + 255) / 256), dim3(256), 0, 0, n, tab, d, d + 0x7C00);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(est, d, n, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(code, d + 0x7C00, n, hipMemcpyDeviceToHost);
   (void)hipFree(d);
   return (int)e;
 }

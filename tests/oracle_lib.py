@@ -46,6 +46,13 @@ def _declare(lib):
     lib.oracle_decode_block_bc6h_float.argtypes = [_vp, _vp, _sz, _i32]
     lib.oracle_decompress_rgba8.argtypes = [_i32, _u32, _u32, _vp, _sz, _vp, _sz]
     lib.oracle_decompress_rgba8.restype = _i32
+    for n in ("16f", "32f"):
+        getattr(lib, f"oracle_decompress_rgba{n}").argtypes = [_i32, _u32, _u32, _vp, _sz, _vp, _sz]
+        getattr(lib, f"oracle_decompress_rgba{n}").restype = _i32
+    lib.oracle_linear_to_srgb.argtypes = [ctypes.c_float]
+    lib.oracle_dds_header.argtypes = [_u32, _u32, _u32, _u32, _u32, _u32, _vp]
+    lib.oracle_dds_header.restype = _i32
+    lib.oracle_linear_to_srgb.restype = ctypes.c_uint8
     lib.oracle_partial_sort_list.argtypes = [_vp, _sz, _u32]
     lib.oracle_get_tables.argtypes = [_vp, _vp, _vp, _vp, _vp]
     lib.oracle_get_partition_sets.argtypes = [_vp]
@@ -175,6 +182,44 @@ def decompress_rgba8(tag: int, width: int, height: int, blocks: np.ndarray) -> n
     if rc != 0:
         raise ValueError(f"oracle_decompress_rgba8 failed: {rc}")
     return out.reshape(height, width, 4)
+
+
+def _decompress_float(name, dtype, tag, width, height, blocks, fill):
+    blocks = np.ascontiguousarray(blocks, dtype=np.uint8)
+    out = np.empty(width * height * 4, dtype)
+    out.view(np.uint16 if dtype == np.float16 else np.uint32)[:] = fill
+    rc = getattr(lib(), name)(tag, width, height, _ptr(blocks), blocks.size, _ptr(out), out.size)
+    if rc != 0:
+        raise ValueError(f"{name} failed: {rc}")
+    return out.reshape(height, width, 4)
+
+
+def decompress_rgba16f(tag: int, width: int, height: int, blocks: np.ndarray, fill: int = 0) -> np.ndarray:
+    """decompress_blocks_as_rgba16f (decode.rs:397-430) into a buffer whose
+    every element starts as the bit pattern `fill`; the reference's layout
+    (RGB at stride 3 inside each block's RGBA span) is kept."""
+    return _decompress_float("oracle_decompress_rgba16f", np.float16, tag, width, height, blocks, fill)
+
+
+def decompress_rgba32f(tag: int, width: int, height: int, blocks: np.ndarray, fill: int = 0) -> np.ndarray:
+    """decompress_blocks_as_rgba32f (decode.rs:438-468), as above."""
+    return _decompress_float("oracle_decompress_rgba32f", np.float32, tag, width, height, blocks, fill)
+
+
+def linear_to_srgb(values) -> np.ndarray:
+    """linear_to_srgb (decode.rs:151-159) of each f32 value."""
+    f = lib().oracle_linear_to_srgb
+    return np.array([f(float(v)) for v in np.asarray(values, np.float32).ravel()], np.uint8)
+
+
+def dds_header(dxgi_format: int, width: int, height: int, mips: int = 1, layers: int = 1,
+               alpha_mode: int = 1) -> bytes:
+    """The 148 bytes ddsfile 0.5 writes before the data (oracle/dds.cpp)."""
+    out = np.zeros(148, np.uint8)
+    rc = lib().oracle_dds_header(dxgi_format, width, height, mips, layers, alpha_mode, _ptr(out))
+    if rc != 0:
+        raise ValueError(f"oracle_dds_header failed: {rc}")
+    return out.tobytes()
 
 
 def tables():

@@ -17,7 +17,7 @@ HEADER = os.path.join(os.path.dirname(os.path.dirname(__file__)), "include", "bc
 
 def declared_functions():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|void|size_t|const char\*)\s+(bcn_\w+)\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void|size_t|uint32_t|const char\*)\s+(bcn_\w+)\(", text, re.M)))
 
 
 def test_library_exports_every_declared_symbol():

@@ -1,0 +1,150 @@
+"""The image-level GPU decoders (decompress_blocks_as_rgba8 / _rgba16f /
+_rgba32f, src/decode.rs:336-468) against the oracle.
+
+* the BC6H -> RGBA8 sRGB8 code equals the oracle's linear_to_srgb on every
+  non-negative finite half (exhaustive, through the kernel's own device
+  function), and its hardware estimate is within one code everywhere;
+* BC6H -> RGBA8 images (random blocks and a GPU-encoded HDR image) equal the
+  oracle's decompress_blocks_as_rgba8 byte for byte;
+* RGBA16F / RGBA32F with the reference layout equal the oracle element for
+  element, including the untouched elements; the default RGBA layout holds
+  the same RGB values with alpha 1.0;
+* non-BC6H variants are rejected for the float outputs.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib
+from block_compression_amd import synth
+from block_compression_amd.settings import BC6HSettings, BC7Settings, CompressionVariant
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+FASTMATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fastmath", "_build",
+                        "libfastmath_check.so")
+V6 = CompressionVariant.BC6H(BC6HSettings.basic())
+
+
+def test_srgb8_code_exhaustive_on_every_half():
+    from block_compression_amd import decode
+
+    print("decode-image: sRGB8 code sweep", flush=True)
+    lib = ctypes.CDLL(FASTMATH)
+    t = decode.bc6h_srgb8_thresholds()
+    n = 0x7C00
+    est = np.zeros(n, np.uint8)
+    code = np.zeros(n, np.uint8)
+    rc = lib.srgb8_check(t.ctypes.data_as(ctypes.c_void_p), ctypes.c_uint32(n),
+                         est.ctypes.data_as(ctypes.c_void_p), code.ctypes.data_as(ctypes.c_void_p))
+    assert rc == 0
+    want = oracle_lib.linear_to_srgb(np.arange(n, dtype=np.uint16).view(np.float16).astype(np.float32))
+    assert np.abs(est.astype(int) - want.astype(int)).max() <= 1
+    assert np.array_equal(code, want)
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _encoded_hdr(w, h):
+    from block_compression_amd import GpuBlockCompressor
+
+    img = synth.hdr_image_rgba16f(w, h, 5)
+    buf = torch.zeros(V6.blocks_byte_size(w, h), dtype=torch.uint8, device="cuda")
+    c = GpuBlockCompressor()
+    c.add_compression_task(V6, _dev(img), w, h, buf)
+    c.compress()
+    return buf
+
+
+@pytest.mark.parametrize("source", ["random", "encoded"])
+def test_bc6h_rgba8_matches_oracle(source):
+    from block_compression_amd import decode
+
+    print(f"decode-image: BC6H rgba8 {source}", flush=True)
+    w, h = (256, 128) if source == "random" else (128, 128)
+    if source == "random":
+        blocks = _dev(np.random.default_rng(31).integers(0, 256, V6.blocks_byte_size(w, h), dtype=np.uint8))
+    else:
+        blocks = _encoded_hdr(w, h)
+    out = torch.full((h, w, 4), 0xAB, dtype=torch.uint8, device="cuda")
+    decode.decompress_blocks_as_rgba8(V6, w, h, blocks, out)
+    torch.cuda.synchronize()
+    want = oracle_lib.decompress_rgba8(5, w, h, blocks.cpu().numpy())
+    assert np.array_equal(out.cpu().numpy(), want)
+
+
+@pytest.mark.parametrize("kind", ["16f", "32f"])
+@pytest.mark.parametrize("source", ["random", "encoded"])
+def test_bc6h_float_images_match_oracle(kind, source):
+    from block_compression_amd import decode
+
+    print(f"decode-image: BC6H rgba{kind} {source}", flush=True)
+    w, h = 96, 64
+    if source == "random":
+        blocks = _dev(np.random.default_rng(37).integers(0, 256, V6.blocks_byte_size(w, h), dtype=np.uint8))
+    else:
+        blocks = _encoded_hdr(w, h)
+    nb = blocks.cpu().numpy()
+    if kind == "16f":
+        dt, it, fill, fn = torch.float16, np.uint16, 0x7E00, decode.decompress_blocks_as_rgba16f
+        want = oracle_lib.decompress_rgba16f(5, w, h, nb, fill).view(it)
+        one = 0x3C00
+    else:
+        dt, it, fill, fn = torch.float32, np.uint32, 0x7FC00001, decode.decompress_blocks_as_rgba32f
+        want = oracle_lib.decompress_rgba32f(5, w, h, nb, fill).view(it)
+        one = 0x3F800000
+    tdt = torch.int16 if kind == "16f" else torch.int32
+    # Reference layout over a NaN-filled buffer: element for element.
+    out = torch.full((h, w, 4), fill, dtype=tdt, device="cuda").view(dt)
+    fn(V6, w, h, blocks, out, reference_layout=True)
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy().view(it), want)
+    # Default RGBA layout: the same RGB per texel, alpha 1.0.
+    rgba = torch.empty((h, w, 4), dtype=dt, device="cuda")
+    fn(V6, w, h, blocks, rgba)
+    torch.cuda.synchronize()
+    got = rgba.cpu().numpy().view(it)
+    rows = want.reshape(h, w // 4, 16)[:, :, :12].reshape(h, w, 3)
+    assert np.array_equal(got[..., :3], rows)
+    assert np.all(got[..., 3] == one)
+
+
+def test_float_outputs_reject_other_variants():
+    from block_compression_amd import BcnError, decode
+
+    v = CompressionVariant.BC7(BC7Settings.alpha_basic())
+    blocks = torch.zeros(v.blocks_byte_size(8, 8), dtype=torch.uint8, device="cuda")
+    with pytest.raises(BcnError, match="UNSUPPORTED"):
+        decode.decompress_blocks_as_rgba16f(v, 8, 8, blocks, torch.zeros((8, 8, 4), dtype=torch.float16, device="cuda"))
+    with pytest.raises(BcnError, match="UNSUPPORTED"):
+        decode.decompress_blocks_as_rgba32f(v, 8, 8, blocks, torch.zeros((8, 8, 4), dtype=torch.float32, device="cuda"))
+
+
+def test_bc6h_2048_float_and_srgb_consistent():
+    """At the 2048^2 HDR config: the RGBA32F decode is the exact widening of
+    the RGBA16F decode, and a seeded sample of RGBA8 rows equals the oracle."""
+    from block_compression_amd import decode
+
+    print("decode-image: BC6H 2048 consistency", flush=True)
+    w = h = 2048
+    blocks = _encoded_hdr(w, h)
+    f16 = torch.empty((h, w, 4), dtype=torch.float16, device="cuda")
+    f32 = torch.empty((h, w, 4), dtype=torch.float32, device="cuda")
+    s8 = torch.empty((h, w, 4), dtype=torch.uint8, device="cuda")
+    decode.decompress_blocks_as_rgba16f(V6, w, h, blocks, f16)
+    decode.decompress_blocks_as_rgba32f(V6, w, h, blocks, f32)
+    decode.decompress_blocks_as_rgba8(V6, w, h, blocks, s8)
+    torch.cuda.synchronize()
+    assert torch.equal(f16.float(), f32)
+    rows = np.random.default_rng(41).choice(h // 4, 16, replace=False)
+    nb = blocks.cpu().numpy().reshape(h // 4, -1)
+    got = s8.cpu().numpy()
+    for by in rows:
+        want = oracle_lib.decompress_rgba8(5, w, 4, nb[by])
+        assert np.array_equal(got[by * 4:by * 4 + 4], want)
