@@ -31,6 +31,10 @@ STATUS_NAMES = {
 
 SRC_RGBA8 = 0
 SRC_RGBA16F = 1
+SRC_R8 = 2
+SRC_RG8 = 3
+SRC_BGRA8 = 4
+SRC_RGBA32F = 5
 DECODE_REFERENCE_LAYOUT = 1
 
 # (name, restype, argtypes) of every exported symbol.

@@ -7,9 +7,10 @@ Mirrors `GpuBlockCompressor` (src/block_compressor.rs:45-62):
                                     texture_y_offset=None, blocks_offset=None)  # (:342)
     compressor.compress(stream=None)                            # (:540)
 
-`texture` is a CUDA/HIP torch tensor of shape (H, W, 4): uint8 (Rgba8Unorm)
-or float16 (Rgba16Float, BC6H only); `buffer` is a contiguous uint8 torch
-tensor on the same device.  The argument checks are the reference's asserts
+`texture` is a device tensor (or a DLPack-exporting array) whose dtype and
+channel count give the texture view's format: uint8 RGBA8 / BGRA8 / RG8 / R8
+for BC1-BC5 and BC7, float16 or float32 RGBA for BC6H; `buffer` is a
+contiguous uint8 torch tensor on the same device.  The argument checks are the reference's asserts
 (block_compressor.rs:352-373), raised as BcnError.  compress() enqueues the
 kernels on `stream` (default: torch's current stream) and empties the queue.
 torch is used only to hand over device pointers and streams.
@@ -62,21 +63,22 @@ class GpuBlockCompressor:
         buffer,
         texture_y_offset: Optional[int] = None,
         blocks_offset: Optional[int] = None,
+        texture_format: Optional[str] = None,
     ) -> None:
+        """add_compression_task (src/block_compressor.rs:342-387).
+
+        `texture` is a device tensor, or any object with `__dlpack__` (imported
+        without a copy), of shape (H, W, C) or (H, W): its layout is the
+        texture view's format (see texture_layout).  Rows may have any pitch
+        (a row-strided view of a larger texture is fine); texels within a row
+        must be packed.  `buffer` is a contiguous uint8 device tensor."""
         import torch
 
         if not isinstance(variant, CompressionVariant):
             raise TypeError("variant must be a CompressionVariant")
-        if texture.dim() != 3 or texture.shape[2] != 4:
-            raise ValueError("texture must have shape (height, width, 4)")
-        if texture.dtype == torch.uint8:
-            fmt = _capi.SRC_RGBA8
-        elif texture.dtype == torch.float16:
-            fmt = _capi.SRC_RGBA16F
-        else:
-            raise TypeError("texture must be uint8 (RGBA8) or float16 (RGBA16F)")
-        if texture.stride(2) != 1 or texture.stride(1) != 4:
-            raise ValueError("texture texels must be packed (stride (pitch, 4, 1))")
+        texture = import_texture(texture)
+        buffer = import_texture(buffer)
+        fmt = texture_layout(texture, texture_format)
         if buffer.dtype != torch.uint8 or not buffer.is_contiguous():
             raise TypeError("buffer must be a contiguous uint8 tensor")
         if not texture.is_cuda or not buffer.is_cuda:
@@ -122,6 +124,66 @@ class GpuBlockCompressor:
         """Moves every queued task into a BlockCompressionBatch (one persistent
         launch over all of them; see bcn_batch_create)."""
         return BlockCompressionBatch(self, config)
+
+
+# Source layouts by name (bcn_source_format).
+SOURCE_FORMATS = {
+    "rgba8": _capi.SRC_RGBA8,
+    "bgra8": _capi.SRC_BGRA8,
+    "r8": _capi.SRC_R8,
+    "rg8": _capi.SRC_RG8,
+    "rgba16f": _capi.SRC_RGBA16F,
+    "rgba32f": _capi.SRC_RGBA32F,
+}
+
+
+def import_texture(obj):
+    """A torch tensor as is; any other object exporting `__dlpack__` (CuPy,
+    JAX, another framework's device array) through torch.from_dlpack, which
+    shares its memory."""
+    import torch
+
+    if isinstance(obj, torch.Tensor):
+        return obj
+    if hasattr(obj, "__dlpack__"):
+        return torch.from_dlpack(obj)
+    raise TypeError(f"expected a torch tensor or a DLPack-exporting array, got {type(obj).__name__}")
+
+
+def texture_layout(texture, texture_format: Optional[str] = None) -> int:
+    """The bcn_source_format of a texture tensor, from its dtype and channel
+    count, the formats of the views the reference accepts
+    (src/block_compressor.rs:300-313):
+
+        uint8   (H, W) or (H, W, 1) -> R8      uint8   (H, W, 2) -> RG8
+        uint8   (H, W, 4)           -> RGBA8   (or "bgra8" when named)
+        float16 (H, W, 4)           -> RGBA16F float32 (H, W, 4) -> RGBA32F
+
+    `texture_format` names the layout explicitly (a key of SOURCE_FORMATS);
+    it must agree with the tensor's dtype and channel count."""
+    import torch
+
+    if texture.dim() == 2:
+        channels = 1
+    elif texture.dim() == 3:
+        channels = int(texture.shape[2])
+    else:
+        raise ValueError("texture must have shape (height, width[, channels])")
+    inferred = {(torch.uint8, 1): "r8", (torch.uint8, 2): "rg8", (torch.uint8, 4): "rgba8",
+                (torch.float16, 4): "rgba16f", (torch.float32, 4): "rgba32f"}.get((texture.dtype, channels))
+    if inferred is None:
+        raise TypeError(f"unsupported texture: dtype {texture.dtype} with {channels} channels")
+    name = inferred if texture_format is None else texture_format.lower()
+    if name not in SOURCE_FORMATS:
+        raise ValueError(f"unknown texture_format {texture_format!r}")
+    if name != inferred and not (name == "bgra8" and inferred == "rgba8"):
+        raise ValueError(f"texture_format {name!r} does not match a {inferred} tensor")
+    # Texels packed within a row; rows at any pitch.
+    if texture.dim() == 3 and texture.stride(2) != 1:
+        raise ValueError("texture channels must be packed")
+    if texture.stride(1) != channels:
+        raise ValueError("texture texels must be packed within a row")
+    return SOURCE_FORMATS[name]
 
 
 def _stream_handle(device, stream) -> int:

@@ -24,6 +24,7 @@
 #include "bc15_block.hpp"
 #include "bc7_block.hpp"
 #include "bcn_internal.hpp"
+#include "texel_load.hpp"
 
 namespace bcn {
 
@@ -35,6 +36,7 @@ struct alignas(16) QueueTask {
   uint32_t bw;             // blocks per row
   uint32_t variant;        // bcn_variant: BC1..BC5 or BC7
   uint32_t vec;            // rows 16-byte aligned and blocks aligned to their size
+  uint32_t src;            // bcn_source_format (an 8-bit layout)
   Bc7Params s;             // BC7 settings (unused by BC1-BC5)
 };
 
@@ -44,29 +46,6 @@ struct alignas(16) QueueChunk {
 };
 
 constexpr int kQueueWave = 64;
-
-__device__ __forceinline__ void queue_load_block(const uint8_t* tex, uint32_t pitch, uint32_t bw,
-                                                 uint32_t idx, bool vec, uint32_t px[16]) {
-  uint32_t yy = idx / bw, xx = idx - yy * bw;
-  const uint8_t* p = tex + (size_t)yy * 4 * pitch + (size_t)xx * 16;
-  if (vec) {
-#pragma unroll
-    for (int y = 0; y < 4; y++) {
-      uint4 v = *reinterpret_cast<const uint4*>(p + (size_t)y * pitch);
-      px[y * 4 + 0] = v.x;
-      px[y * 4 + 1] = v.y;
-      px[y * 4 + 2] = v.z;
-      px[y * 4 + 3] = v.w;
-    }
-  } else {
-#pragma unroll
-    for (int y = 0; y < 4; y++) {
-      const uint32_t* r = reinterpret_cast<const uint32_t*>(p + (size_t)y * pitch);
-#pragma unroll
-      for (int x = 0; x < 4; x++) px[y * 4 + x] = r[x];
-    }
-  }
-}
 
 // Work-queue classes, one persistent kernel each.  Keeping the BC7 search for
 // 3 and 4 channels and the BC1-BC5 encoders in separate functions lets each
@@ -89,7 +68,8 @@ __device__ __forceinline__ void queue_encode_chunk(const QueueTask* t, const Que
     const bool active = b + lane < ch.count;
     const uint32_t idx = ch.first + (active ? b + lane : 0u);
     uint32_t px[16];
-    queue_load_block(tex, pitch, bw, idx, vec, px);
+    const uint32_t yy = idx / bw, xx = idx - yy * bw;
+    load_block_rgba8(tex, pitch, xx, yy, t->src, vec, px);
     uint32_t out[4] = {0u, 0u, 0u, 0u};
     bool half = false;
     if (CLASS == kQueueBc7Rgba) {
@@ -367,6 +347,7 @@ int bcn_batch_create(bcn_compressor* c, const bcn_batch_config* config, bcn_batc
     en.q.pitch = t.pitch;
     en.q.bw = bcn::blocks_x(t);
     en.q.variant = (uint32_t)t.variant;
+    en.q.src = (uint32_t)t.format;
     const uintptr_t align = (t.variant == BCN_BC1 || t.variant == BCN_BC4) ? 7u : 15u;
     en.q.vec = ((((uintptr_t)t.texture | (uintptr_t)t.pitch) & 15u) == 0 &&
                 ((uintptr_t)t.blocks & align) == 0)

@@ -1,12 +1,14 @@
 // BC1-BC5 encode kernels: one lane per 4x4 block, row-major block order.
 //
 // Replace compress_bc1..compress_bc5 (src/shader/bc1_to_5.wgsl:454-591).
-// Every lane reads its block as four 16-byte row loads (adjacent lanes read
-// adjacent 16-byte chunks) and writes 8 or 16 bytes.  BC4/BC5 touch 64 bytes
-// of RGBA8 input per block for 8/16 bytes out and are HBM-bound; BC1-BC3 add
-// a short PCA fit per block.
+// Every lane reads its block as four row loads (16 bytes for RGBA8; adjacent
+// lanes read adjacent chunks; texel_load.hpp for the other layouts) and
+// writes 8 or 16 bytes.  BC4/BC5 touch 64 bytes of RGBA8 input per block (16
+// of R8, 32 of RG8) for 8/16 bytes out and are HBM-bound; BC1-BC3 add a short
+// PCA fit per block.
 #include "bc15_block.hpp"
 #include "bcn_internal.hpp"
+#include "texel_load.hpp"
 
 namespace bcn {
 
@@ -14,28 +16,14 @@ constexpr int kBc15Group = 256;
 
 template <int FMT, bool VEC>
 __global__ __launch_bounds__(kBc15Group) void bc15_encode_kernel(const uint8_t* __restrict__ src,
-                                                                 uint32_t pitch, uint32_t bw,
-                                                                 uint32_t nblocks,
+                                                                 uint32_t pitch, uint32_t src_format,
+                                                                 uint32_t bw, uint32_t nblocks,
                                                                  uint32_t* __restrict__ dst) {
   uint32_t idx = blockIdx.x * kBc15Group + threadIdx.x;
   if (idx >= nblocks) return;
   uint32_t yy = idx / bw, xx = idx - yy * bw;
-  const uint8_t* p = src + (size_t)yy * 4 * pitch + (size_t)xx * 16;
   uint32_t px[16];
-#pragma unroll
-  for (int y = 0; y < 4; y++) {
-    if (VEC) {
-      uint4 v = *reinterpret_cast<const uint4*>(p + (size_t)y * pitch);
-      px[y * 4 + 0] = v.x;
-      px[y * 4 + 1] = v.y;
-      px[y * 4 + 2] = v.z;
-      px[y * 4 + 3] = v.w;
-    } else {
-      const uint32_t* r = reinterpret_cast<const uint32_t*>(p + (size_t)y * pitch);
-#pragma unroll
-      for (int x = 0; x < 4; x++) px[y * 4 + x] = r[x];
-    }
-  }
+  load_block_rgba8(src, pitch, xx, yy, src_format, VEC, px);
   constexpr int WORDS = (FMT == BCN_BC1 || FMT == BCN_BC4) ? 2 : 4;
   uint32_t out[4];
   if (FMT == BCN_BC1) {
@@ -69,9 +57,9 @@ static hipError_t launch_fmt(const Task& t, uint32_t bw, uint32_t nblocks, bool 
   dim3 grid((nblocks + kBc15Group - 1) / kBc15Group), block(kBc15Group);
   uint32_t* dst = reinterpret_cast<uint32_t*>(t.blocks);
   if (vec)
-    hipLaunchKernelGGL((bc15_encode_kernel<FMT, true>), grid, block, 0, stream, t.texture, t.pitch, bw, nblocks, dst);
+    hipLaunchKernelGGL((bc15_encode_kernel<FMT, true>), grid, block, 0, stream, t.texture, t.pitch, (uint32_t)t.format, bw, nblocks, dst);
   else
-    hipLaunchKernelGGL((bc15_encode_kernel<FMT, false>), grid, block, 0, stream, t.texture, t.pitch, bw, nblocks, dst);
+    hipLaunchKernelGGL((bc15_encode_kernel<FMT, false>), grid, block, 0, stream, t.texture, t.pitch, (uint32_t)t.format, bw, nblocks, dst);
   return hipGetLastError();
 }
 

@@ -1,39 +1,28 @@
 // BC6H (unsigned) encode kernel: one lane per 4x4 block, row-major order.
 //
 // Replaces compress_bc6h (src/shader/bc6h.wgsl:1401-1424).  Each lane reads
-// its block of RGBA16F texels as four 32-byte rows and keeps the 48 scaled
-// channel values in registers; the 32 partition keys live in LDS, one row
-// per lane, interleaved by lane.
+// its block of RGBA16F texels as four 32-byte rows (RGBA32F: 64-byte rows,
+// rounded to halves) and keeps the 48 scaled channel values in registers;
+// the 32 partition keys live in LDS, one row per lane, interleaved by lane.
 #include "bc6h_block.hpp"
 #include "bcn_internal.hpp"
+#include "texel_load.hpp"
 
 namespace bcn {
 
 constexpr int kBc6hGroup = 64;
 
 __global__ __launch_bounds__(kBc6hGroup) void bc6h_encode_kernel(const uint8_t* __restrict__ src,
-                                                                 uint32_t pitch, uint32_t bw,
-                                                                 uint32_t nblocks,
+                                                                 uint32_t pitch, uint32_t src_format,
+                                                                 uint32_t bw, uint32_t nblocks,
                                                                  uint32_t* __restrict__ dst,
                                                                  Bc6hParams s) {
   __shared__ int keys[32 * kBc6hGroup];
   uint32_t idx = blockIdx.x * kBc6hGroup + threadIdx.x;
   if (idx >= nblocks) return;
   uint32_t yy = idx / bw, xx = idx - yy * bw;
-  const uint8_t* p = src + (size_t)yy * 4 * pitch + (size_t)xx * 32;
   uint16_t bits[48];
-#pragma unroll
-  for (int y = 0; y < 4; y++) {
-    const uint2* row = reinterpret_cast<const uint2*>(p + (size_t)y * pitch);
-#pragma unroll
-    for (int x = 0; x < 4; x++) {
-      uint2 t = row[x];  // R | G << 16, B | A << 16
-      int k = y * 4 + x;
-      bits[k] = (uint16_t)(t.x & 0xFFFFu);
-      bits[16 + k] = (uint16_t)(t.x >> 16);
-      bits[32 + k] = (uint16_t)(t.y & 0xFFFFu);
-    }
-  }
+  load_block_rgb16f(src, pitch, xx, yy, src_format, bits);
   uint32_t out[4];
   encode_bc6h_block(bits, s, keys + threadIdx.x, kBc6hGroup, out);
   uint32_t* o = dst + (size_t)idx * 4;
@@ -42,15 +31,17 @@ __global__ __launch_bounds__(kBc6hGroup) void bc6h_encode_kernel(const uint8_t* 
 }
 
 hipError_t launch_bc6h(const Task& t, hipStream_t stream) {
-  if (t.format != BCN_SRC_RGBA16F) return hipErrorNotSupported;
+  if (t.format != BCN_SRC_RGBA16F && t.format != BCN_SRC_RGBA32F) return hipErrorNotSupported;
   Bc6hParams s;
   static_assert(sizeof(Bc6hParams) == 20, "BC6HSettings is 5 x u32");
   __builtin_memcpy(&s, t.settings, sizeof(s));
   uint32_t bw = blocks_x(t), nblocks = bw * blocks_y(t);
   if (nblocks == 0) return hipSuccess;
-  if ((((uintptr_t)t.texture | (uintptr_t)t.pitch) & 7u) != 0) return hipErrorInvalidValue;
+  const uintptr_t align = t.format == BCN_SRC_RGBA32F ? 15u : 7u;
+  if ((((uintptr_t)t.texture | (uintptr_t)t.pitch) & align) != 0) return hipErrorInvalidValue;
   dim3 grid((nblocks + kBc6hGroup - 1) / kBc6hGroup), block(kBc6hGroup);
-  hipLaunchKernelGGL(bc6h_encode_kernel, grid, block, 0, stream, t.texture, t.pitch, bw, nblocks,
+  hipLaunchKernelGGL(bc6h_encode_kernel, grid, block, 0, stream, t.texture, t.pitch,
+                     (uint32_t)t.format, bw, nblocks,
                      reinterpret_cast<uint32_t*>(t.blocks), s);
   return hipGetLastError();
 }

@@ -9,6 +9,7 @@
 // LDS as 16-bit bounds (8 KiB per 64-lane workgroup), interleaved by lane.
 #include "bc7_block.hpp"
 #include "bcn_internal.hpp"
+#include "texel_load.hpp"
 
 namespace bcn {
 
@@ -18,30 +19,16 @@ constexpr int kBc7Wave = 64;
 template <int CH, bool VEC>
 __global__ __launch_bounds__(kBc7Wave)
 __attribute__((amdgpu_waves_per_eu(BCN_BC7_WAVES, BCN_BC7_WAVES))) void bc7_encode_kernel(const uint8_t* __restrict__ src,
-                                                              uint32_t pitch, uint32_t bw,
-                                                              uint32_t nblocks,
+                                                              uint32_t pitch, uint32_t src_format,
+                                                              uint32_t bw, uint32_t nblocks,
                                                               uint32_t* __restrict__ dst,
                                                               Bc7Params s) {
   __shared__ uint16_t keys[64 * kBc7Wave];
   uint32_t idx = blockIdx.x * kBc7Wave + threadIdx.x;
   if (idx >= nblocks) return;
   uint32_t yy = idx / bw, xx = idx - yy * bw;
-  const uint8_t* p = src + (size_t)yy * 4 * pitch + (size_t)xx * 16;
   uint32_t px[16];
-#pragma unroll
-  for (int y = 0; y < 4; y++) {
-    if (VEC) {
-      uint4 v = *reinterpret_cast<const uint4*>(p + (size_t)y * pitch);
-      px[y * 4 + 0] = v.x;
-      px[y * 4 + 1] = v.y;
-      px[y * 4 + 2] = v.z;
-      px[y * 4 + 3] = v.w;
-    } else {
-      const uint32_t* r = reinterpret_cast<const uint32_t*>(p + (size_t)y * pitch);
-#pragma unroll
-      for (int x = 0; x < 4; x++) px[y * 4 + x] = r[x];
-    }
-  }
+  load_block_rgba8(src, pitch, xx, yy, src_format, VEC, px);
   uint32_t out[4];
   encode_bc7_block<CH>(px, s, keys + threadIdx.x, kBc7Wave, out);
   uint32_t* o = dst + (size_t)idx * 4;
@@ -63,11 +50,11 @@ hipError_t launch_bc7(const Task& t, hipStream_t stream) {
   bool vec = (((uintptr_t)t.texture | (uintptr_t)t.pitch | (uintptr_t)t.blocks) & 15u) == 0;
   uint32_t* dst = reinterpret_cast<uint32_t*>(t.blocks);
   if (s.channels == 4) {
-    if (vec) hipLaunchKernelGGL((bc7_encode_kernel<4, true>), grid, block, 0, stream, t.texture, t.pitch, bw, nblocks, dst, s);
-    else hipLaunchKernelGGL((bc7_encode_kernel<4, false>), grid, block, 0, stream, t.texture, t.pitch, bw, nblocks, dst, s);
+    if (vec) hipLaunchKernelGGL((bc7_encode_kernel<4, true>), grid, block, 0, stream, t.texture, t.pitch, (uint32_t)t.format, bw, nblocks, dst, s);
+    else hipLaunchKernelGGL((bc7_encode_kernel<4, false>), grid, block, 0, stream, t.texture, t.pitch, (uint32_t)t.format, bw, nblocks, dst, s);
   } else {
-    if (vec) hipLaunchKernelGGL((bc7_encode_kernel<3, true>), grid, block, 0, stream, t.texture, t.pitch, bw, nblocks, dst, s);
-    else hipLaunchKernelGGL((bc7_encode_kernel<3, false>), grid, block, 0, stream, t.texture, t.pitch, bw, nblocks, dst, s);
+    if (vec) hipLaunchKernelGGL((bc7_encode_kernel<3, true>), grid, block, 0, stream, t.texture, t.pitch, (uint32_t)t.format, bw, nblocks, dst, s);
+    else hipLaunchKernelGGL((bc7_encode_kernel<3, false>), grid, block, 0, stream, t.texture, t.pitch, (uint32_t)t.format, bw, nblocks, dst, s);
   }
   return hipGetLastError();
 }

@@ -9,6 +9,7 @@
 
 #include "bcn_common.hpp"
 #include "bcn_internal.hpp"
+#include "texel_load.hpp"
 
 namespace {
 
@@ -84,15 +85,16 @@ int bcn_add_compression_task(bcn_compressor* c, bcn_variant variant, const void*
     return fail(c, BCN_ERR_NOT_MULTIPLE_OF_4, "width and height must be multiples of 4");
   if (texture_y_offset % 4 != 0)
     return fail(c, BCN_ERR_NOT_MULTIPLE_OF_4, "texture_y_offset must be a multiple of 4");
-  if (texture_format != BCN_SRC_RGBA8 && texture_format != BCN_SRC_RGBA16F)
-    return fail(c, BCN_ERR_INVALID_ARGUMENT, "unknown texture format");
-  if (texture_format == BCN_SRC_RGBA16F && variant != BCN_BC6H)
-    return fail(c, BCN_ERR_UNSUPPORTED, "an RGBA16F source is only accepted for BC6H");
+  const uint32_t texel_bytes = bcn::source_texel_bytes(texture_format);
+  if (texel_bytes == 0) return fail(c, BCN_ERR_INVALID_ARGUMENT, "unknown texture format");
+  const bool float_src = texture_format == BCN_SRC_RGBA16F || texture_format == BCN_SRC_RGBA32F;
+  if (float_src && variant != BCN_BC6H)
+    return fail(c, BCN_ERR_UNSUPPORTED, "a float source is only accepted for BC6H");
   // The reference's RGBA8 route into BC6H (compress_rgba8, encode.rs:130-137)
   // converts through powf-based sRGB decoding; the GPU path takes half-float
-  // texels only (compress_rgba16, encode.rs:211-251).
-  if (texture_format == BCN_SRC_RGBA8 && variant == BCN_BC6H)
-    return fail(c, BCN_ERR_UNSUPPORTED, "BC6H takes an RGBA16F source");
+  // (or float) texels only (compress_rgba16, encode.rs:211-251).
+  if (!float_src && variant == BCN_BC6H)
+    return fail(c, BCN_ERR_UNSUPPORTED, "BC6H takes an RGBA16F or RGBA32F source");
   bcn::Task t{};
   t.variant = variant;
   if (variant == BCN_BC7) {
@@ -122,7 +124,6 @@ int bcn_add_compression_task(bcn_compressor* c, bcn_variant variant, const void*
   } else if (!texture || !buffer) {
     return fail(c, BCN_ERR_INVALID_ARGUMENT, "null texture or buffer");
   }
-  uint32_t texel_bytes = texture_format == BCN_SRC_RGBA16F ? 8u : 4u;
   if ((uint64_t)texture_pitch < (uint64_t)width * texel_bytes)
     return fail(c, BCN_ERR_INVALID_ARGUMENT, "texture_pitch smaller than a row of the region");
   if ((uint64_t)texture_y_offset + height > texture_height)
@@ -133,6 +134,9 @@ int bcn_add_compression_task(bcn_compressor* c, bcn_variant variant, const void*
     return fail(c, BCN_ERR_BUFFER_TOO_SMALL, "buffer too small for the blocks at blocks_offset");
   if ((((uintptr_t)texture | (uintptr_t)buffer | texture_pitch) & 3u) != 0)
     return fail(c, BCN_ERR_MISALIGNED, "texture, buffer and texture_pitch must be 4-byte aligned");
+  // Float texels are read whole (8 or 16 bytes at a time).
+  if (float_src && (((uintptr_t)texture | texture_pitch) & (texel_bytes - 1)) != 0)
+    return fail(c, BCN_ERR_MISALIGNED, "float texture and texture_pitch must be aligned to a texel");
   t.texture = static_cast<const uint8_t*>(texture) + (size_t)texture_y_offset * texture_pitch;
   t.format = texture_format;
   t.pitch = texture_pitch;

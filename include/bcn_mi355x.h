@@ -34,11 +34,21 @@ typedef enum bcn_variant {
   BCN_BC7 = 6   /* RGB(A), 16 bytes; settings: bcn_bc7_settings */
 } bcn_variant;
 
-/* Source texel formats (the reference's texture views: Rgba8Unorm and
- * Rgba16Float, src/block_compressor.rs:301-313). */
+/* Source texel layouts: the texture views the reference's shaders read with
+ * textureLoad (src/block_compressor.rs:300-313).  The encoders work on the
+ * raw texel values: an sRGB texture is passed as its bytes (the reference's
+ * non-sRGB view of it).  A channel a layout lacks reads as textureLoad gives
+ * it: G = B = 0, A = 255.  BC1-BC5 and BC7 take the 8-bit layouts; BC6H
+ * takes RGBA16F or RGBA32F. */
 typedef enum bcn_source_format {
-  BCN_SRC_RGBA8 = 0,   /* 4 x u8 per texel, read as exact integers */
-  BCN_SRC_RGBA16F = 1  /* 4 x IEEE binary16 per texel (BC6H only) */
+  BCN_SRC_RGBA8 = 0,    /* Rgba8Unorm: 4 x u8 per texel, read as exact integers */
+  BCN_SRC_RGBA16F = 1,  /* Rgba16Float: 4 x IEEE binary16 per texel (BC6H) */
+  BCN_SRC_R8 = 2,       /* R8Unorm: 1 byte per texel -> (r, 0, 0, 255) */
+  BCN_SRC_RG8 = 3,      /* Rg8Unorm: 2 bytes per texel -> (r, g, 0, 255) */
+  BCN_SRC_BGRA8 = 4,    /* Bgra8Unorm: 4 bytes per texel, B first */
+  BCN_SRC_RGBA32F = 5   /* Rgba32Float: 4 x f32 per texel (BC6H); each channel is
+                           rounded to the nearest half (ties to even) and every
+                           zero reads as +0, as bc6h.wgsl:59-70 converts it */
 } bcn_source_format;
 
 typedef enum bcn_status {
