@@ -58,8 +58,10 @@ def psnr_rgb(orig, dec) -> float:
 
 def bench_formats(img, reps: int = 5, launches: int = 20) -> dict:
     """The other kernels on their configurations: BC1/BC3/BC4/BC5 encode of the
-    same 4096x4096 texture, BC6H 'slow' encode of the 2048x2048 HDR texture,
-    and the GPU decode of the 4096x4096 BC7 output.  Each figure is the best of
+    same 4096x4096 texture (BC4 also from its R8 channel, BC5 from RG8), BC6H
+    'slow' encode of the 2048x2048 HDR texture, the GPU decode of the
+    4096x4096 BC7 output and the RGBA16F / RGBA32F / sRGB8 decodes of the BC6H
+    output.  Each figure is the best of
     `reps` runs of `launches` back-to-back launches between two HIP events,
     divided by `launches` (one launch of BC4 is ~10 us, comparable to launch
     gaps).  HBM roofline against 8 TB/s with the algorithmic bytes per block
@@ -135,6 +137,35 @@ def bench_formats(img, reps: int = 5, launches: int = 20) -> dict:
 
     rund()
     out[f"bc7_decode_{W}"] = entry(timed(rund), W, H, 80)
+    # BC4 / BC5 from single- and two-channel sources (R8, RG8): 16 / 32 bytes
+    # of texels per block instead of 64.
+    for name, v, src in [("bc4_encode_r8", CompressionVariant.BC4(), tex[..., 0].contiguous()),
+                         ("bc5_encode_rg8", CompressionVariant.BC5(), tex[..., :2].contiguous())]:
+        buf = torch.empty(v.blocks_byte_size(W, H), dtype=torch.uint8, device="cuda")
+
+        def runs(v=v, buf=buf, src=src):
+            comp.add_compression_task(v, src, W, H, buf)
+            comp.compress()
+
+        runs()
+        texel_bytes = 16 * src.element_size() * (1 if src.dim() == 2 else src.shape[2])
+        out[f"{name}_{W}"] = entry(timed(runs), W, H, texel_bytes + v.block_byte_size())
+    # Image-level decodes of the BC6H blocks (2048^2): RGBA16F, RGBA32F and
+    # sRGB8 (exact linear_to_srgb).
+    for name, fn, dt, tb in [("rgba16f", "bcn_decode_rgba16f", torch.float16, 8),
+                             ("rgba32f", "bcn_decode_rgba32f", torch.float32, 16),
+                             ("srgb8", "bcn_decode_rgba8", torch.uint8, 4)]:
+        img_out = torch.empty((2048, 2048, 4), dtype=dt, device="cuda")
+        f = getattr(lib(), fn)
+
+        def rundh(f=f, img_out=img_out, tb=tb, fn=fn):
+            args = [5, b6.data_ptr(), b6.numel(), 2048, 2048, img_out.data_ptr(), 2048 * tb]
+            if fn != "bcn_decode_rgba8":
+                args.append(0)
+            check(f(*args, ctypes.c_void_p(stream.cuda_stream)))
+
+        rundh()
+        out[f"bc6h_decode_{name}_2048"] = entry(timed(rundh), 2048, 2048, 16 + 16 * tb)
     torch.cuda.synchronize()
     return out
 
