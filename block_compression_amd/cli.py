@@ -75,7 +75,7 @@ def read_image(file_name: str, variant: CompressionVariant) -> np.ndarray:
 
     with Image.open(file_name) as im:
         rgba = np.array(im.convert("RGBA"), dtype=np.uint8)
-    if variant.tag == CompressionVariant.BC6H(BC6HSettings.basic()).tag:
+    if variant.is_bc6h:
         return srgb_to_linear_f16(rgba)
     return np.ascontiguousarray(rgba)
 

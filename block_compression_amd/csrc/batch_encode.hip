@@ -336,7 +336,7 @@ int bcn_batch_create(bcn_compressor* c, const bcn_batch_config* config, bcn_batc
   std::vector<Entry> entries;
   for (const bcn::Task& t : tasks) {
     uint32_t nb = bcn::blocks_x(t) * bcn::blocks_y(t);
-    if (t.variant == BCN_BC6H) {
+    if (t.variant == BCN_BC6H || t.variant == BCN_BC6H_SF16) {
       b->per_task.push_back(t);
       continue;
     }

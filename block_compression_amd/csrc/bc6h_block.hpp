@@ -331,20 +331,50 @@ struct Bc6hState {
   int qlo[3], qhi[3];  // qbounds
 };
 
+// Signed (SF16) encoding -- not in the reference, whose encoder is unsigned
+// only (its decoder reads signed blocks, decode/block.rs:266, 697-719).  The
+// same search over the signed value domain: a texel of half bits h is
+// +-(|h| / 31) * 32 (the inverse of the signed finish_unquantize,
+// block.rs:1391-1397); an endpoint's magnitude quantises to 0..2^(b-1)-1 over
+// 0..32767 as the unsigned one does over 0..65535, and dequantises as the
+// decoder's signed unquantize (block.rs:1369-1389); the interpolation is the
+// decoder's floor (fblock_quant's >> 6).  Endpoints are stored as 16-bit two's
+// complement, so every packed field (within the low 16 bits) is exact.
+BCN_HD int quant_signed(float e, int bits) {
+  const int top = (1 << (bits - 1)) - 1;
+  int m = qclamp(fabsf(e) / 32767.0f * (float)top + 0.5f, 0.0f, (float)top);
+  return e < 0.0f ? -m : m;
+}
+
+BCN_HD int unquant_signed(int v, int bits) {
+  if (bits >= 16) return v;
+  int m = v < 0 ? -v : v, u;
+  if (m == 0)
+    u = 0;
+  else if (m >= (1 << (bits - 1)) - 1)
+    u = 0x7FFF;
+  else
+    u = ((m << 15) + 0x4000) >> (bits - 1);
+  return v < 0 ? -u : u;
+}
+
 // ep_quant_dequant_bc6h (bc6h.rs:213-239) of one endpoint.
+template <bool SGN>
 BCN_HD void bc6h_qd(const Bc6hState& st, const float e[3], int q[3], int d[3]) {
   const int levels = 1 << st.epb;
 #pragma unroll
   for (int p = 0; p < 3; p++) {
-    int v = qclamp(div65535(e[p]) * (float)(levels - 1) + 0.5f, 0.0f,
-                   (float)(levels - 1));
+    int v = SGN ? quant_signed(e[p], st.epb)
+                : qclamp(div65535(e[p]) * (float)(levels - 1) + 0.5f, 0.0f,
+                         (float)(levels - 1));
     v = iclamp(v, st.qlo[p], st.qhi[p]);
     q[p] = v;
-    d[p] = unpack_to_uf16(v, st.epb);
+    d[p] = SGN ? unquant_signed(v, st.epb) : unpack_to_uf16(v, st.epb);
   }
 }
 
 // compute_qbounds_core (bc6h.rs:175-185) with per-channel spans.
+template <bool SGN>
 BCN_HD void bc6h_qbounds(Bc6hState& st, const float span[3]) {
   const int levels = 1 << st.epb;
 #pragma unroll
@@ -353,10 +383,15 @@ BCN_HD void bc6h_qbounds(Bc6hState& st, const float span[3]) {
     float lo = middle - span[p] / 2.0f;
     float hi = middle + span[p] / 2.0f;
     // ep_quant_bc6h_8 (bc6h.rs:166-173)
-    st.qlo[p] = qclamp(div65535(lo) * (float)(levels - 1) + 0.5f, 0.0f,
-                       (float)(levels - 1));
-    st.qhi[p] = qclamp(div65535(hi) * (float)(levels - 1) + 0.5f, 0.0f,
-                       (float)(levels - 1));
+    if (SGN) {
+      st.qlo[p] = quant_signed(lo, st.epb);
+      st.qhi[p] = quant_signed(hi, st.epb);
+    } else {
+      st.qlo[p] = qclamp(div65535(lo) * (float)(levels - 1) + 0.5f, 0.0f,
+                         (float)(levels - 1));
+      st.qhi[p] = qclamp(div65535(hi) * (float)(levels - 1) + 0.5f, 0.0f,
+                         (float)(levels - 1));
+    }
   }
 }
 
@@ -373,7 +408,7 @@ BCN_HD void store_qep(uint32_t w[6], const int q[4][3]) {
 #pragma unroll
   for (int i = 0; i < 6; i++) {
     int a = (2 * i) / 3, pa = (2 * i) % 3, b2 = (2 * i + 1) / 3, pb = (2 * i + 1) % 3;
-    w[i] = (uint32_t)q[a][pa] | ((uint32_t)q[b2][pb] << 16);
+    w[i] = ((uint32_t)q[a][pa] & 0xFFFFu) | ((uint32_t)q[b2][pb] << 16);
   }
 }
 
@@ -384,6 +419,7 @@ BCN_HD int load_qep(const uint32_t w[6], int e, int p) {
 
 // bc6h_enc_2p_part_fast + bc6h_enc_2p_list (bc6h.rs:304-388) over the
 // candidate partitions in keys[0..count).
+template <bool SGN>
 BCN_HD void bc6h_enc_2p(const Bc6hBlock& b, const Bc6hParams& s, const Bc6hState& st,
                         const int* keys, int stride, int count, Bc6hBest& best) {
   if (count == 0) return;
@@ -400,8 +436,8 @@ BCN_HD void bc6h_enc_2p(const Bc6hBlock& b, const Bc6hParams& s, const Bc6hState
     for (int j = 0; j < 2; j++) {
       float e0[3], e1[3];
       fsegment(b, part_mask(part, j), e0, e1);
-      bc6h_qd(st, e0, q[2 * j], d[j][0]);
-      bc6h_qd(st, e1, q[2 * j + 1], d[j][1]);
+      bc6h_qd<SGN>(st, e0, q[2 * j], d[j][0]);
+      bc6h_qd<SGN>(st, e1, q[2 * j + 1], d[j][1]);
     }
     uint32_t qb[2];
     float err = fblock_quant<3, 2>(b, d, pattern, qb);
@@ -423,8 +459,8 @@ BCN_HD void bc6h_enc_2p(const Bc6hBlock& b, const Bc6hParams& s, const Bc6hState
     for (int j = 0; j < 2; j++) {
       float e0[3], e1[3];
       fopt_endpoints<3>(b, bqb, part_mask(best_part, j), e0, e1);
-      bc6h_qd(st, e0, q[2 * j], d[j][0]);
-      bc6h_qd(st, e1, q[2 * j + 1], d[j][1]);
+      bc6h_qd<SGN>(st, e0, q[2 * j], d[j][0]);
+      bc6h_qd<SGN>(st, e1, q[2 * j + 1], d[j][1]);
     }
     uint32_t qb[2];
     float err = fblock_quant<3, 2>(b, d, kPart.subset[best_part], qb);
@@ -449,21 +485,22 @@ BCN_HD void bc6h_enc_2p(const Bc6hBlock& b, const Bc6hParams& s, const Bc6hState
 }
 
 // bc6h_enc_1p (bc6h.rs:390-412).
+template <bool SGN>
 BCN_HD void bc6h_enc_1p(const Bc6hBlock& b, const Bc6hParams& s, const Bc6hState& st,
                         Bc6hBest& best) {
   float e0[3], e1[3];
   fsegment(b, 0xFFFFu, e0, e1);
   int q[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
   int d[2][2][3];
-  bc6h_qd(st, e0, q[0], d[0][0]);
-  bc6h_qd(st, e1, q[1], d[0][1]);
+  bc6h_qd<SGN>(st, e0, q[0], d[0][0]);
+  bc6h_qd<SGN>(st, e1, q[1], d[0][1]);
   uint32_t qb[2];
   float err = fblock_quant<4, 1>(b, d, 0u, qb);
 #pragma unroll 1
   for (uint32_t it = 0; it < s.refine_iterations_1p; it++) {
     fopt_endpoints<4>(b, qb, 0xFFFFu, e0, e1);
-    bc6h_qd(st, e0, q[0], d[0][0]);
-    bc6h_qd(st, e1, q[1], d[0][1]);
+    bc6h_qd<SGN>(st, e0, q[0], d[0][0]);
+    bc6h_qd<SGN>(st, e1, q[1], d[0][1]);
     err = fblock_quant<4, 1>(b, d, 0u, qb);
   }
   if (err < best.err) {
@@ -510,6 +547,7 @@ BCN_HD void bc6h_keys(const Bc6hBlock& b, int* keys, int stride, int count) {
 }
 
 // bc6h_test_mode (bc6h.rs:414-446).  Returns false if the span test skips it.
+template <bool SGN>
 BCN_HD bool bc6h_set_mode(Bc6hState& st, int m, float margin) {
   float span = bc6h_span(m);
   if (st.max_span * margin > span) return false;
@@ -521,7 +559,7 @@ BCN_HD bool bc6h_set_mode(Bc6hState& st, int m, float margin) {
     st.mode = m + st.max_span_idx;
     if (st.max_span_idx < 3) sp[st.max_span_idx] *= 2.0f;
   }
-  bc6h_qbounds(st, sp);
+  bc6h_qbounds<SGN>(st, sp);
   return true;
 }
 
@@ -603,7 +641,8 @@ BCN_HD void bc6h_code(const Bc6hBest& best, uint32_t out[4]) {
 }
 
 // compress_bc6h_core (bc6h.rs:821-860) for one block of f16 bit patterns
-// (channel p of texel k at bits[p*16+k]).
+// (channel p of texel k at bits[p*16+k]); SGN: the signed (SF16) encoding.
+template <bool SGN>
 BCN_HD void encode_bc6h_block(const uint16_t bits[48], const Bc6hParams& s, int* keys, int stride,
                               uint32_t out[4]) {
   Bc6hBlock b;
@@ -611,13 +650,20 @@ BCN_HD void encode_bc6h_block(const uint16_t bits[48], const Bc6hParams& s, int*
   // bc6h_setup (bc6h.rs:793-819)
   for (int p = 0; p < 3; p++) {
     st.rgb_bounds[p] = 65535.0f;
-    st.rgb_bounds[3 + p] = 0.0f;
+    st.rgb_bounds[3 + p] = SGN ? -65535.0f : 0.0f;
   }
 #pragma unroll
   for (int p = 0; p < 3; p++)
 #pragma unroll
     for (int k = 0; k < 16; k++) {
-      float v = ((float)bits[p * 16 + k] / 31.0f) * 64.0f;
+      const uint32_t h = bits[p * 16 + k];
+      float v;
+      if (SGN) {
+        v = ((float)(h & 0x7FFFu) / 31.0f) * 32.0f;
+        if (h & 0x8000u) v = -v;
+      } else {
+        v = ((float)h / 31.0f) * 64.0f;
+      }
       b.blk[p * 16 + k] = v;
       st.rgb_bounds[p] = fminf(st.rgb_bounds[p], v);
       st.rgb_bounds[3 + p] = fmaxf(st.rgb_bounds[3 + p], v);
@@ -646,36 +692,37 @@ BCN_HD void encode_bc6h_block(const uint16_t bits[48], const Bc6hParams& s, int*
     const int two_p[6] = {0, 1, 2, 5, 6, 9};
 #pragma unroll 1
     for (int i = 0; i < 6; i++) {
-      if (!bc6h_set_mode(st, two_p[i], 0.0f)) continue;
+      if (!bc6h_set_mode<SGN>(st, two_p[i], 0.0f)) continue;
       if (!keys_ready) {
         bc6h_keys(b, keys, stride, count);
         keys_ready = true;
       }
-      bc6h_enc_2p(b, s, st, keys, stride, count, best);
+      bc6h_enc_2p<SGN>(b, s, st, keys, stride, count, best);
     }
 #pragma unroll 1
     for (int m = 10; m < 14; m++)
-      if (bc6h_set_mode(st, m, 0.0f)) bc6h_enc_1p(b, s, st, best);
+      if (bc6h_set_mode<SGN>(st, m, 0.0f)) bc6h_enc_1p<SGN>(b, s, st, best);
   } else {
     if (s.fast_skip_threshold > 0) {
       // The span tests only move the current mode; the last one that passes
       // is the mode bc6h_enc_2p encodes with (bc6h.rs:836-851).
-      bc6h_set_mode(st, 9, 0.0f);
-      if (s.fast_mode != 0) bc6h_set_mode(st, 1, 1.0f);
-      bc6h_set_mode(st, 6, 1.0f / 1.2f);
-      bc6h_set_mode(st, 5, 1.0f / 1.2f);
-      bc6h_set_mode(st, 0, 1.0f / 1.2f);
-      bc6h_set_mode(st, 2, 1.0f);
+      bc6h_set_mode<SGN>(st, 9, 0.0f);
+      if (s.fast_mode != 0) bc6h_set_mode<SGN>(st, 1, 1.0f);
+      bc6h_set_mode<SGN>(st, 6, 1.0f / 1.2f);
+      bc6h_set_mode<SGN>(st, 5, 1.0f / 1.2f);
+      bc6h_set_mode<SGN>(st, 0, 1.0f / 1.2f);
+      bc6h_set_mode<SGN>(st, 2, 1.0f);
       bc6h_keys(b, keys, stride, count);
       keys_ready = true;
-      bc6h_enc_2p(b, s, st, keys, stride, count, best);
-      if (s.fast_mode == 0 && bc6h_set_mode(st, 1, 0.0f)) bc6h_enc_2p(b, s, st, keys, stride, count, best);
+      bc6h_enc_2p<SGN>(b, s, st, keys, stride, count, best);
+      if (s.fast_mode == 0 && bc6h_set_mode<SGN>(st, 1, 0.0f))
+        bc6h_enc_2p<SGN>(b, s, st, keys, stride, count, best);
     }
-    bc6h_set_mode(st, 10, 0.0f);
-    bc6h_set_mode(st, 11, 1.0f);
-    bc6h_set_mode(st, 12, 1.0f);
-    bc6h_set_mode(st, 13, 1.0f);
-    bc6h_enc_1p(b, s, st, best);
+    bc6h_set_mode<SGN>(st, 10, 0.0f);
+    bc6h_set_mode<SGN>(st, 11, 1.0f);
+    bc6h_set_mode<SGN>(st, 12, 1.0f);
+    bc6h_set_mode<SGN>(st, 13, 1.0f);
+    bc6h_enc_1p<SGN>(b, s, st, best);
   }
   if (best.mode < 0) {
     out[0] = out[1] = out[2] = out[3] = 0;

@@ -12,6 +12,7 @@ namespace bcn {
 
 constexpr int kBc6hGroup = 64;
 
+template <bool SGN>
 __global__ __launch_bounds__(kBc6hGroup) void bc6h_encode_kernel(const uint8_t* __restrict__ src,
                                                                  uint32_t pitch, uint32_t src_format,
                                                                  uint32_t bw, uint32_t nblocks,
@@ -24,7 +25,7 @@ __global__ __launch_bounds__(kBc6hGroup) void bc6h_encode_kernel(const uint8_t* 
   uint16_t bits[48];
   load_block_rgb16f(src, pitch, xx, yy, src_format, bits);
   uint32_t out[4];
-  encode_bc6h_block(bits, s, keys + threadIdx.x, kBc6hGroup, out);
+  encode_bc6h_block<SGN>(bits, s, keys + threadIdx.x, kBc6hGroup, out);
   uint32_t* o = dst + (size_t)idx * 4;
 #pragma unroll
   for (int i = 0; i < 4; i++) o[i] = out[i];
@@ -40,9 +41,13 @@ hipError_t launch_bc6h(const Task& t, hipStream_t stream) {
   const uintptr_t align = t.format == BCN_SRC_RGBA32F ? 15u : 7u;
   if ((((uintptr_t)t.texture | (uintptr_t)t.pitch) & align) != 0) return hipErrorInvalidValue;
   dim3 grid((nblocks + kBc6hGroup - 1) / kBc6hGroup), block(kBc6hGroup);
-  hipLaunchKernelGGL(bc6h_encode_kernel, grid, block, 0, stream, t.texture, t.pitch,
-                     (uint32_t)t.format, bw, nblocks,
-                     reinterpret_cast<uint32_t*>(t.blocks), s);
+  uint32_t* dst = reinterpret_cast<uint32_t*>(t.blocks);
+  if (t.variant == BCN_BC6H_SF16)
+    hipLaunchKernelGGL(bc6h_encode_kernel<true>, grid, block, 0, stream, t.texture, t.pitch,
+                       (uint32_t)t.format, bw, nblocks, dst, s);
+  else
+    hipLaunchKernelGGL(bc6h_encode_kernel<false>, grid, block, 0, stream, t.texture, t.pitch,
+                       (uint32_t)t.format, bw, nblocks, dst, s);
   return hipGetLastError();
 }
 

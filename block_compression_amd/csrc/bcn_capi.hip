@@ -20,14 +20,17 @@ int fail(bcn_compressor* c, int status, const char* what) {
 
 uint32_t block_bytes(bcn_variant v) { return (v == BCN_BC1 || v == BCN_BC4) ? 8u : 16u; }
 
-bool valid_variant(int v) { return v >= BCN_BC1 && v <= BCN_BC7; }
+bool valid_variant(int v) { return v >= BCN_BC1 && v <= BCN_BC6H_SF16; }
+
+bool is_bc6h(int v) { return v == BCN_BC6H || v == BCN_BC6H_SF16; }
 
 }  // namespace
 
 hipError_t bcn::launch_task(const bcn::Task& t, hipStream_t stream) {
   switch (t.variant) {
     case BCN_BC7: return bcn::launch_bc7(t, stream);
-    case BCN_BC6H: return bcn::launch_bc6h(t, stream);
+    case BCN_BC6H:
+    case BCN_BC6H_SF16: return bcn::launch_bc6h(t, stream);
     default: return bcn::launch_bc15(t, stream);
   }
 }
@@ -88,12 +91,12 @@ int bcn_add_compression_task(bcn_compressor* c, bcn_variant variant, const void*
   const uint32_t texel_bytes = bcn::source_texel_bytes(texture_format);
   if (texel_bytes == 0) return fail(c, BCN_ERR_INVALID_ARGUMENT, "unknown texture format");
   const bool float_src = texture_format == BCN_SRC_RGBA16F || texture_format == BCN_SRC_RGBA32F;
-  if (float_src && variant != BCN_BC6H)
+  if (float_src && !is_bc6h(variant))
     return fail(c, BCN_ERR_UNSUPPORTED, "a float source is only accepted for BC6H");
   // The reference's RGBA8 route into BC6H (compress_rgba8, encode.rs:130-137)
   // converts through powf-based sRGB decoding; the GPU path takes half-float
   // (or float) texels only (compress_rgba16, encode.rs:211-251).
-  if (!float_src && variant == BCN_BC6H)
+  if (!float_src && is_bc6h(variant))
     return fail(c, BCN_ERR_UNSUPPORTED, "BC6H takes an RGBA16F or RGBA32F source");
   bcn::Task t{};
   t.variant = variant;
@@ -108,7 +111,7 @@ int bcn_add_compression_task(bcn_compressor* c, bcn_variant variant, const void*
         s.fast_skip_threshold_mode7 > 64)
       return fail(c, BCN_ERR_INVALID_ARGUMENT, "BC7 settings: partition thresholds exceed 64");
     memcpy(t.settings, &s, sizeof(s));
-  } else if (variant == BCN_BC6H) {
+  } else if (is_bc6h(variant)) {
     if (!settings || settings_size != sizeof(bcn_bc6h_settings))
       return fail(c, BCN_ERR_INVALID_ARGUMENT, "BC6H needs a bcn_bc6h_settings");
     bcn_bc6h_settings s;

@@ -70,6 +70,7 @@ uint32_t bcn_dxgi_format(bcn_variant variant) {
     case BCN_BC4: return 80;   // BC4_UNORM
     case BCN_BC5: return 83;   // BC5_UNORM
     case BCN_BC6H: return 95;  // BC6H_UF16
+    case BCN_BC6H_SF16: return 96;  // BC6H_SF16 (the signed extension)
     case BCN_BC7: return 99;   // BC7_UNORM_SRGB
     default: return 0;
   }

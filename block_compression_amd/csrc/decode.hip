@@ -77,7 +77,9 @@ __global__ __launch_bounds__(kDecGroup) void decode_rgba8_kernel(const uint8_t* 
 }
 
 // BC6H -> RGBA8 (decode.rs:161-181): unsigned decode, linear_to_srgb per
-// channel, alpha 0.
+// channel, alpha 0.  SGN: signed blocks, whose negative values (and -0) map
+// to code 0 as linear_to_srgb clamps them.
+template <bool SGN>
 __global__ __launch_bounds__(kDecGroup) void decode_bc6h_srgb8_kernel(
     const uint8_t* __restrict__ blocks, uint32_t bw, uint32_t nblocks, uint8_t* __restrict__ rgba,
     uint32_t pitch, Srgb8Table tab) {
@@ -89,17 +91,19 @@ __global__ __launch_bounds__(kDecGroup) void decode_bc6h_srgb8_kernel(
   const uint32_t* src = reinterpret_cast<const uint32_t*>(blocks + (size_t)idx * 16);
   uint32_t in[4] = {src[0], src[1], src[2], src[3]};
   uint16_t h[48];
-  decode_bc6h(in, false, h);
+  decode_bc6h(in, SGN, h);
   uint32_t yy = idx / bw, xx = idx - yy * bw;
   uint8_t* o = rgba + (size_t)yy * 4 * pitch + (size_t)xx * 16;
+  auto code = [&](uint16_t v) -> uint32_t {
+    return (SGN && (v & 0x8000u)) ? 0u : srgb8_code(v, lt);
+  };
 #pragma unroll
   for (int y = 0; y < 4; y++) {
     uint32_t px[4];
 #pragma unroll
     for (int x = 0; x < 4; x++) {
       int k = y * 4 + x;
-      px[x] = srgb8_code(h[k * 3], lt) | (srgb8_code(h[k * 3 + 1], lt) << 8) |
-              (srgb8_code(h[k * 3 + 2], lt) << 16);
+      px[x] = code(h[k * 3]) | (code(h[k * 3 + 1]) << 8) | (code(h[k * 3 + 2]) << 16);
     }
     *reinterpret_cast<uint4*>(o + (size_t)y * pitch) = make_uint4(px[0], px[1], px[2], px[3]);
   }
@@ -198,12 +202,14 @@ const Srgb8Table& srgb8_table() {
 
 bool launch_ok() { return hipGetLastError() == hipSuccess; }
 
-// decompress_blocks_as_rgba16f / _rgba32f (decode.rs:397-468): BC6H only.
+// decompress_blocks_as_rgba16f / _rgba32f (decode.rs:397-468): BC6H only
+// (BC6H_SF16 blocks decode signed).
 int decode_bc6h_image(bcn_variant variant, const void* blocks, size_t blocks_size, uint32_t width,
                       uint32_t height, int is_signed, void* rgba, uint32_t rgba_pitch,
                       uint32_t flags, bool f32, void* hip_stream) {
-  if (variant < BCN_BC1 || variant > BCN_BC7) return BCN_ERR_INVALID_ARGUMENT;
-  if (variant != BCN_BC6H) return BCN_ERR_UNSUPPORTED;
+  if (variant < BCN_BC1 || variant > BCN_BC6H_SF16) return BCN_ERR_INVALID_ARGUMENT;
+  if (variant != BCN_BC6H && variant != BCN_BC6H_SF16) return BCN_ERR_UNSUPPORTED;
+  if (variant == BCN_BC6H_SF16) is_signed = 1;
   if (flags & ~(uint32_t)BCN_DECODE_REFERENCE_LAYOUT) return BCN_ERR_INVALID_ARGUMENT;
   if (width % 4 || height % 4) return BCN_ERR_NOT_MULTIPLE_OF_4;
   uint32_t bw = width / 4, nblocks = bw * (height / 4);
@@ -235,7 +241,7 @@ extern "C" {
 
 int bcn_decode_rgba8(bcn_variant variant, const void* blocks, size_t blocks_size, uint32_t width,
                      uint32_t height, void* rgba, uint32_t rgba_pitch, void* hip_stream) {
-  if (variant < BCN_BC1 || variant > BCN_BC7) return BCN_ERR_INVALID_ARGUMENT;
+  if (variant < BCN_BC1 || variant > BCN_BC6H_SF16) return BCN_ERR_INVALID_ARGUMENT;
   if (width % 4 || height % 4) return BCN_ERR_NOT_MULTIPLE_OF_4;
   size_t bb = (variant == BCN_BC1 || variant == BCN_BC4) ? 8 : 16;
   uint32_t bw = width / 4, nblocks = bw * (height / 4);
@@ -255,7 +261,8 @@ int bcn_decode_rgba8(bcn_variant variant, const void* blocks, size_t blocks_size
     case BCN_BC3: hipLaunchKernelGGL(decode_rgba8_kernel<BCN_BC3>, grid, block, 0, s, b, bw, nblocks, o, rgba_pitch); break;
     case BCN_BC4: hipLaunchKernelGGL(decode_rgba8_kernel<BCN_BC4>, grid, block, 0, s, b, bw, nblocks, o, rgba_pitch); break;
     case BCN_BC5: hipLaunchKernelGGL(decode_rgba8_kernel<BCN_BC5>, grid, block, 0, s, b, bw, nblocks, o, rgba_pitch); break;
-    case BCN_BC6H: hipLaunchKernelGGL(decode_bc6h_srgb8_kernel, grid, block, 0, s, b, bw, nblocks, o, rgba_pitch, srgb8_table()); break;
+    case BCN_BC6H: hipLaunchKernelGGL(decode_bc6h_srgb8_kernel<false>, grid, block, 0, s, b, bw, nblocks, o, rgba_pitch, srgb8_table()); break;
+    case BCN_BC6H_SF16: hipLaunchKernelGGL(decode_bc6h_srgb8_kernel<true>, grid, block, 0, s, b, bw, nblocks, o, rgba_pitch, srgb8_table()); break;
     default: hipLaunchKernelGGL(decode_rgba8_kernel<BCN_BC7>, grid, block, 0, s, b, bw, nblocks, o, rgba_pitch); break;
   }
   return launch_ok() ? BCN_OK : BCN_ERR_HIP;

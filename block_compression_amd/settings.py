@@ -162,9 +162,10 @@ BC6H_PRESETS = {
     "very_slow": BC6HSettings.very_slow,
 }
 
-# Format tags of the C ABI (include/bcn_mi355x.h), in lib.rs:74-103 order.
-BC1, BC2, BC3, BC4, BC5, BC6H, BC7 = range(7)
-_NAMES = ("bc1", "bc2", "bc3", "bc4", "bc5", "bc6h", "bc7")
+# Format tags of the C ABI (include/bcn_mi355x.h), in lib.rs:74-103 order,
+# then the signed BC6H extension (BC6H_SF16: not a variant of the reference).
+BC1, BC2, BC3, BC4, BC5, BC6H, BC7, BC6H_SF16 = range(8)
+_NAMES = ("bc1", "bc2", "bc3", "bc4", "bc5", "bc6h", "bc7", "bc6h_sf16")
 
 
 class CompressionVariant:
@@ -173,11 +174,13 @@ class CompressionVariant:
     __slots__ = ("tag", "settings")
 
     def __init__(self, tag: int, settings: Optional[Union[BC6HSettings, BC7Settings]] = None):
-        if tag == BC6H and not isinstance(settings, BC6HSettings):
+        if tag not in range(len(_NAMES)):
+            raise ValueError(f"unknown variant tag {tag}")
+        if tag in (BC6H, BC6H_SF16) and not isinstance(settings, BC6HSettings):
             raise TypeError("BC6H needs BC6HSettings")
         if tag == BC7 and not isinstance(settings, BC7Settings):
             raise TypeError("BC7 needs BC7Settings")
-        if tag not in (BC6H, BC7) and settings is not None:
+        if tag not in (BC6H, BC6H_SF16, BC7) and settings is not None:
             raise TypeError("only BC6H and BC7 take settings")
         self.tag = tag
         self.settings = settings
@@ -210,6 +213,16 @@ class CompressionVariant:
     @staticmethod
     def BC7(settings: BC7Settings) -> "CompressionVariant":
         return CompressionVariant(BC7, settings)
+
+    @staticmethod
+    def BC6H_SF16(settings: BC6HSettings) -> "CompressionVariant":
+        """Signed BC6H (DXGI BC6H_SF16): the BC6H search over signed half
+        floats.  An extension: the reference encodes unsigned BC6H only."""
+        return CompressionVariant(BC6H_SF16, settings)
+
+    @property
+    def is_bc6h(self) -> bool:
+        return self.tag in (BC6H, BC6H_SF16)
 
     # lib.rs:105-120: discriminant-only equality and hash.
     def __eq__(self, other) -> bool:

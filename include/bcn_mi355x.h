@@ -31,7 +31,14 @@ typedef enum bcn_variant {
   BCN_BC4 = 3,  /* R, 8 bytes */
   BCN_BC5 = 4,  /* RG, 16 bytes */
   BCN_BC6H = 5, /* RGB half float (unsigned), 16 bytes; settings: bcn_bc6h_settings */
-  BCN_BC7 = 6   /* RGB(A), 16 bytes; settings: bcn_bc7_settings */
+  BCN_BC7 = 6,  /* RGB(A), 16 bytes; settings: bcn_bc7_settings */
+  /* RGB half float, signed (BC6H_SF16), 16 bytes; settings: bcn_bc6h_settings.
+   * No counterpart in the reference's encoder (unsigned only); its decoder
+   * reads these blocks (src/decode/block.rs:266, 697-719).  The unsigned
+   * search over signed values: a texel +-(|h|/31)*32, endpoint magnitudes
+   * quantised to 2^(b-1)-1 levels over 0..32767 and dequantised as the
+   * decoder's signed unquantize, two's-complement endpoint fields. */
+  BCN_BC6H_SF16 = 7
 } bcn_variant;
 
 /* Source texel layouts: the texture views the reference's shaders read with
@@ -141,7 +148,8 @@ size_t bcn_blocks_byte_size(bcn_variant variant, uint32_t width, uint32_t height
 /* GPU decode of a block image into RGBA8, the layout of
  * decompress_blocks_as_rgba8 (src/decode.rs:336-387): BC4 -> (R,0,0,0),
  * BC5 -> (R,G,0,0), BC6H -> (sRGB8(R), sRGB8(G), sRGB8(B), 0) with
- * linear_to_srgb (decode.rs:151-181) reproduced exactly.
+ * linear_to_srgb (decode.rs:151-181) reproduced exactly (BC6H_SF16: the
+ * same of the signed decode, negative values giving 0).
  * blocks: device pointer, (w/4)*(h/4) blocks, exactly blocks_byte_size bytes
  * (decode.rs:343-355); rgba: device pointer, rgba_pitch bytes per row, both
  * 16-byte aligned; width and height multiples of 4. */
@@ -158,8 +166,9 @@ int bcn_decode_rgba8(bcn_variant variant, const void* blocks, size_t blocks_size
 #define BCN_DECODE_REFERENCE_LAYOUT 1u
 
 /* decompress_blocks_as_rgba16f (src/decode.rs:397-430) on the GPU: BC6H
- * (unsigned) only, any other variant is BCN_ERR_UNSUPPORTED (the reference
- * panics).  rgba_pitch in bytes, >= width*8. */
+ * (unsigned) and BC6H_SF16 (signed) only, any other variant is
+ * BCN_ERR_UNSUPPORTED (the reference panics).  rgba_pitch in bytes,
+ * >= width*8. */
 int bcn_decode_rgba16f(bcn_variant variant, const void* blocks, size_t blocks_size, uint32_t width,
                        uint32_t height, void* rgba, uint32_t rgba_pitch, uint32_t flags,
                        void* hip_stream);

@@ -84,6 +84,14 @@ struct Bc6h {
   uint32_t epb;
   int32_t qbounds[8];
   const BC6HSettings* settings;
+  // Signed (SF16) encoding.  Not in the reference, whose encoder is unsigned
+  // only; its decoder reads signed blocks (decode/block.rs:266, 697-719).
+  // The same search over the signed value domain: a texel of half bits h is
+  // +-(|h| / 31) * 32 (the inverse of the signed finish_unquantize,
+  // block.rs:1391-1397), endpoints quantise to +-(2^(b-1) - 1) and
+  // dequantise as the decoder's signed unquantize (block.rs:1369-1389), and
+  // endpoint fields are two's complement in b bits.
+  bool sgn = false;
 
   // bc6h.rs:131-135
   static uint32_t get_mode_prefix(size_t mode) {
@@ -107,10 +115,35 @@ struct Bc6h {
     return T[mode];
   }
 
+  // Signed quantisation of one endpoint value: the magnitude rounds to
+  // 0..2^(b-1)-1 over 0..32767 as the unsigned one rounds over 0..65535.
+  static int32_t quant_signed(F e, uint32_t bits) {
+    int32_t top = (1 << (bits - 1)) - 1;
+    int32_t m = iclamp(f2i(fabs_(e) / F(32767.0f) * i2f(top) + F(0.5f)), 0, top);
+    return e < F(0.0f) ? -m : m;
+  }
+
+  // Signed unquantize (block.rs:1369-1389).
+  static int32_t unquant_signed(int32_t v, uint32_t bits) {
+    if (bits >= 16) return v;
+    int32_t m = v < 0 ? -v : v, u;
+    if (m == 0)
+      u = 0;
+    else if (m >= (1 << (bits - 1)) - 1)
+      u = 0x7FFF;
+    else
+      u = ((m << 15) + 0x4000) >> (bits - 1);
+    return v < 0 ? -u : u;
+  }
+
   // bc6h.rs:166-173
   void ep_quant_bc6h_8(const F ep[8], uint32_t bits, uint32_t pairs) {
     int32_t levels = 1 << bits;
     for (size_t i = 0; i < 8 * (size_t)pairs; i++) {
+      if (sgn) {
+        qbounds[i] = quant_signed(ep[i], bits);
+        continue;
+      }
       int32_t v = f2i(ep[i] / F(256.0f * 256.0f - 1.0f) * i2f(levels - 1) + F(0.5f));
       qbounds[i] = iclamp(v, 0, levels - 1);
     }
@@ -148,27 +181,44 @@ struct Bc6h {
   }
 
   // bc6h.rs:213-220
-  static void ep_quant_bc6h(int32_t qep[24], const F ep[24], uint32_t bits, size_t pairs) {
+  static void ep_quant_bc6h(int32_t qep[24], const F ep[24], uint32_t bits, size_t pairs,
+                            bool sgn) {
     int32_t levels = 1 << bits;
     for (size_t i = 0; i < 8 * pairs; i++) {
+      if (sgn) {
+        qep[i] = quant_signed(ep[i], bits);
+        continue;
+      }
       int32_t v = f2i(ep[i] / F(256.0f * 256.0f - 1.0f) * i2f(levels - 1) + F(0.5f));
       qep[i] = iclamp(v, 0, levels - 1);
     }
   }
 
   // bc6h.rs:222-226
-  static void ep_dequant_bc6h(F ep[24], const int32_t qep[24], uint32_t bits, size_t pairs) {
-    for (size_t i = 0; i < 8 * pairs; i++) ep[i] = u2f(unpack_to_uf16((uint32_t)qep[i], bits));
+  static void ep_dequant_bc6h(F ep[24], const int32_t qep[24], uint32_t bits, size_t pairs,
+                              bool sgn) {
+    for (size_t i = 0; i < 8 * pairs; i++)
+      ep[i] = sgn ? i2f(unquant_signed(qep[i], bits))
+                  : u2f(unpack_to_uf16((uint32_t)qep[i], bits));
   }
 
   // bc6h.rs:228-239
   void ep_quant_dequant_bc6h(int32_t qep[24], F ep[24], size_t pairs) const {
     uint32_t bits = epb;
-    ep_quant_bc6h(qep, ep, bits, pairs);
+    ep_quant_bc6h(qep, ep, bits, pairs, sgn);
     for (size_t i = 0; i < 2 * pairs; i++)
       for (int p = 0; p < 3; p++)
         qep[i * 4 + p] = iclamp(qep[i * 4 + p], qbounds[p], qbounds[4 + p]);
-    ep_dequant_bc6h(ep, qep, bits, pairs);
+    ep_dequant_bc6h(ep, qep, bits, pairs, sgn);
+  }
+
+  // Signed endpoints go to the packer as their low `epb` bits (two's
+  // complement): every field it writes lies in those bits, and the deltas
+  // modulo 2^delta_bits are unchanged.
+  void mask_signed(int32_t qep[24], size_t pairs) const {
+    if (!sgn) return;
+    for (size_t i = 0; i < 2 * pairs; i++)
+      for (int p = 0; p < 3; p++) qep[i * 4 + p] &= (1 << epb) - 1;
   }
 
   static uint32_t bit_at(int32_t v, uint32_t pos) { return (uint32_t)((v >> pos) & 1); }
@@ -454,6 +504,7 @@ struct Bc6h {
   void code_2p(int32_t qep[24], const uint32_t qblock[2], int32_t part_id, size_t m) {
     const uint32_t bits = 3;
     uint32_t flips = bc7_code_apply_swap_mode01237(qep, qblock, 1, part_id);
+    mask_signed(qep, 2);
     std::memset(data, 0, sizeof(data));
     uint32_t pos = 0;
     uint32_t packed[4];
@@ -470,6 +521,7 @@ struct Bc6h {
   // bc6h.rs:268-286
   void code_1p(int32_t qep[24], uint32_t qblock[2], size_t m) {
     bc7_code_apply_swap_mode456(qep, 4, qblock, 4);
+    mask_signed(qep, 1);
     std::memset(data, 0, sizeof(data));
     uint32_t pos = 0;
     uint32_t packed[4];
@@ -492,7 +544,7 @@ struct Bc6h {
       block_segment_core(ep + j * 8, block, mask, channels);
     }
     ep_quant_dequant_bc6h(qep, ep, 2);
-    return block_quant(qblock, block, bits, ep, pattern, channels);
+    return block_quant(qblock, block, bits, ep, pattern, channels, sgn);
   }
 
   // bc6h.rs:326-388
@@ -528,7 +580,7 @@ struct Bc6h {
       uint32_t qblock[2] = {0, 0};
       ep_quant_dequant_bc6h(qep, ep, 2);
       uint32_t pattern = get_pattern(best_part_id);
-      F err = block_quant(qblock, block, bits, ep, pattern, channels);
+      F err = block_quant(qblock, block, bits, ep, pattern, channels, sgn);
       if (err < best) {
         std::memcpy(best_qep, qep, sizeof(int32_t) * 8 * pairs);
         best_qblock[0] = qblock[0];
@@ -566,11 +618,11 @@ struct Bc6h {
     int32_t qep[24] = {0};
     ep_quant_dequant_bc6h(qep, ep, 1);
     uint32_t qblock[2] = {0, 0};
-    F err = block_quant(qblock, block, 4, ep, 0, 3);
+    F err = block_quant(qblock, block, 4, ep, 0, 3, sgn);
     for (uint32_t it = 0; it < settings->refine_iterations_1p; it++) {
       opt_endpoints(ep, block, 4, qblock, 0xFFFFFFFFu, 3);
       ep_quant_dequant_bc6h(qep, ep, 1);
-      err = block_quant(qblock, block, 4, ep, 0, 3);
+      err = block_quant(qblock, block, 4, ep, 0, 3, sgn);
     }
     if (err < best_err) {
       best_err = err;
@@ -605,11 +657,19 @@ struct Bc6h {
   void setup() {
     for (int p = 0; p < 3; p++) {
       rgb_bounds[p] = F(65535.0f);
-      rgb_bounds[3 + p] = 0.0f;
+      rgb_bounds[3 + p] = sgn ? F(-65535.0f) : F(0.0f);
     }
     for (int p = 0; p < 3; p++)
       for (int k = 0; k < 16; k++) {
-        F value = (block[p * 16 + k] / F(31.0f)) * F(64.0f);
+        F value;
+        if (sgn) {
+          // Sign and magnitude of the half; -0 maps to -0.0.
+          uint32_t h = (uint32_t)raw(block[p * 16 + k]);
+          value = (u2f(h & 0x7FFFu) / F(31.0f)) * F(32.0f);
+          if (h & 0x8000u) value = -value;
+        } else {
+          value = (block[p * 16 + k] / F(31.0f)) * F(64.0f);
+        }
         block[p * 16 + k] = value;
         rgb_bounds[p] = fmin_(rgb_bounds[p], value);
         rgb_bounds[3 + p] = fmax_(rgb_bounds[3 + p], value);
@@ -683,11 +743,13 @@ struct Bc6h {
 
 }  // namespace
 
-// bc6h.rs:73-97 then encode.rs:601-604
+// bc6h.rs:73-97 then encode.rs:601-604; `is_signed` selects the signed
+// (SF16) encoding described at Bc6h::sgn.
 float bc6h_encode_block_16(const uint16_t* rgba16, size_t stride, const BC6HSettings& s,
-                           uint8_t out[16]) {
+                           uint8_t out[16], bool is_signed) {
   Bc6h st;
   st.init(&s);
+  st.sgn = is_signed;
   for (int y = 0; y < 4; y++)
     for (int x = 0; x < 4; x++) {
       const uint16_t* px = rgba16 + y * stride + x * 4;

@@ -149,7 +149,8 @@ void block_pca_axis(F axis[4], F dc[4], const F block[64], uint32_t mask, size_t
 F block_pca_bound_split(const F block[64], uint32_t mask, const F full_stats[15],
                         size_t channels);
 F block_quant(uint32_t qblock[2], const F block[64], uint32_t bits, const F* ep,
-              uint32_t pattern, size_t channels);
+              uint32_t pattern, size_t channels,
+              bool floor_interp = false);
 void block_segment_core(F* ep, const F block[64], uint32_t mask, size_t channels);
 void bc7_code_qblock(uint32_t data[5], uint32_t* qpos, const uint32_t qblock[2],
                      uint32_t bits, uint32_t flips);
@@ -176,7 +177,7 @@ void bc5_encode_block(const uint8_t* rgba, size_t stride, uint8_t out[16]);
 // BC6H (src/encode/bc6h.rs).  `rgba16` are f16 bit patterns, stride in f16
 // elements.
 float bc6h_encode_block_16(const uint16_t* rgba16, size_t stride, const BC6HSettings& s,
-                           uint8_t out[16]);
+                           uint8_t out[16], bool is_signed = false);
 float bc6h_encode_block_8(const uint8_t* rgba, size_t stride, const BC6HSettings& s,
                           uint8_t out[16]);
 
@@ -187,7 +188,8 @@ void decode_block_bc2(const uint8_t* in, uint8_t* out, size_t pitch);
 void decode_block_bc3(const uint8_t* in, uint8_t* out, size_t pitch);
 void decode_block_bc4(const uint8_t* in, uint8_t* out, size_t pitch);
 void decode_block_bc5(const uint8_t* in, uint8_t* out, size_t pitch);
-void decode_block_bc6h(const uint8_t* in, uint16_t* out, size_t pitch, bool is_signed);
+void decode_block_bc6h(const uint8_t* in, uint16_t* out, size_t pitch, bool is_signed,
+                       int32_t* comp = nullptr);
 void decode_block_bc6h_float(const uint8_t* in, float* out, size_t pitch, bool is_signed);
 void decode_block_bc7(const uint8_t* in, uint8_t* out, size_t pitch);
 

@@ -90,6 +90,29 @@ int oracle_compress_rgba16(const void* settings, const uint16_t* rgba16, uint8_t
   return 0;
 }
 
+// Signed (SF16) BC6H encode of an RGBA16F image; not in the reference (its
+// encoder is unsigned only), see Bc6h::sgn in bc6h.cpp.  Also returns each
+// block's search error in `errs` when non-null.
+int oracle_compress_rgba16_sf16(const void* settings, const uint16_t* rgba16, uint8_t* out,
+                                size_t out_len, uint32_t width, uint32_t height, uint32_t stride,
+                                float* errs, int threads) {
+  if (height % 4 != 0 || width % 4 != 0) return -1;
+  size_t bw = width / 4, bh = height / 4;
+  if (out_len < bw * bh * 16) return -2;
+  BC6HSettings s6;
+  std::memcpy(&s6, settings, sizeof(s6));
+  long nblocks = (long)(bw * bh);
+  if (threads <= 0) threads = 1;
+#pragma omp parallel for schedule(static) num_threads(threads)
+  for (long idx = 0; idx < nblocks; idx++) {
+    size_t yy = (size_t)idx / bw, xx = (size_t)idx % bw;
+    float e = bc6h_encode_block_16(rgba16 + yy * 4 * stride + xx * 16, stride, s6,
+                                   out + (size_t)idx * 16, true);
+    if (errs) errs[idx] = e;
+  }
+  return 0;
+}
+
 // A listed subset of blocks (by block index) of an RGBA8 image: the sampled
 // CPU check of large workloads.  Writes block i's bytes at out + i * bb.
 int oracle_compress_rgba8_sample(int variant, const void* settings, const uint8_t* rgba,
@@ -155,6 +178,10 @@ void oracle_decode_block_bc5(const uint8_t* in, uint8_t* out, size_t pitch) { de
 void oracle_decode_block_bc7(const uint8_t* in, uint8_t* out, size_t pitch) { decode_block_bc7(in, out, pitch); }
 void oracle_decode_block_bc6h(const uint8_t* in, uint16_t* out, size_t pitch, int is_signed) {
   decode_block_bc6h(in, out, pitch, is_signed != 0);
+}
+void oracle_decode_block_bc6h_comp(const uint8_t* in, int32_t* comp, int is_signed) {
+  uint16_t tmp[48];
+  decode_block_bc6h(in, tmp, 12, is_signed != 0, comp);
 }
 void oracle_decode_block_bc6h_float(const uint8_t* in, float* out, size_t pitch, int is_signed) {
   decode_block_bc6h_float(in, out, pitch, is_signed != 0);

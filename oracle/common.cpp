@@ -354,9 +354,11 @@ F block_pca_bound_split(const F block[64], uint32_t mask, const F full_stats[15]
   return fsqrt(bound) * F(256.0f);
 }
 
-// common.rs:422-481
+// common.rs:422-481.  `floor_interp` rounds the interpolation down (>> 6, as
+// the decoder does) instead of Rust's truncating `/ 64`; the two differ only
+// for negative sums, which occur in the signed BC6H extension alone.
 F block_quant(uint32_t qblock[2], const F block[64], uint32_t bits, const F* ep,
-              uint32_t pattern, size_t channels) {
+              uint32_t pattern, size_t channels, bool floor_interp) {
   F total_err = 0.0f;
   int32_t levels = 1 << bits;
   qblock[0] = 0;
@@ -388,8 +390,10 @@ F block_quant(uint32_t qblock[2], const F block[64], uint32_t bits, const F* ep,
     for (size_t p = 0; p < channels; p++) {
       F ep_a = ep[8 * j + p];
       F ep_b = ep[8 * j + 4 + p];
-      F dec_v0 = i2f(((64 - w0) * f2i(ep_a) + w0 * f2i(ep_b) + 32) / 64);
-      F dec_v1 = i2f(((64 - w1) * f2i(ep_a) + w1 * f2i(ep_b) + 32) / 64);
+      int32_t s0 = (64 - w0) * f2i(ep_a) + w0 * f2i(ep_b) + 32;
+      int32_t s1 = (64 - w1) * f2i(ep_a) + w1 * f2i(ep_b) + 32;
+      F dec_v0 = i2f(floor_interp ? s0 >> 6 : s0 / 64);
+      F dec_v1 = i2f(floor_interp ? s1 >> 6 : s1 / 64);
       err0 += sq(dec_v0 - block[k + p * 16]);
       err1 += sq(dec_v1 - block[k + p * 16]);
     }

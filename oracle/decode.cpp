@@ -339,7 +339,11 @@ void decode_block_bc5(const uint8_t* in, uint8_t* out, size_t pitch) {
 }
 
 // block.rs:262-769.  `pitch` is in f16 elements; RGB written at stride 3.
-void decode_block_bc6h(const uint8_t* in, uint16_t* out, size_t pitch, bool is_signed) {
+// `comp`, when non-null, also receives each texel's interpolated value before
+// finish_unquantize (texel k channel c at comp[k*3 + c]): the value the
+// encoder measures its error on.
+void decode_block_bc6h(const uint8_t* in, uint16_t* out, size_t pitch, bool is_signed,
+                       int32_t* comp) {
   static const uint8_t ACTUAL_BITS_COUNT[4][14] = {
       {10, 7, 11, 11, 11, 9, 8, 8, 8, 6, 10, 11, 12, 16},
       {5, 6, 5, 4, 4, 5, 6, 5, 5, 6, 10, 9, 8, 4},
@@ -491,9 +495,13 @@ void decode_block_bc6h(const uint8_t* in, uint16_t* out, size_t pitch, bool is_s
       int32_t index = bs.read_bits_i32(index_bits);
       int ep = ps * 2;
       uint16_t* o = out + i * pitch + j * 3;
-      o[0] = finish_unquantize(interpolate(r[ep], r[ep + 1], weights, index), is_signed);
-      o[1] = finish_unquantize(interpolate(g[ep], g[ep + 1], weights, index), is_signed);
-      o[2] = finish_unquantize(interpolate(b[ep], b[ep + 1], weights, index), is_signed);
+      int32_t v[3] = {interpolate(r[ep], r[ep + 1], weights, index),
+                      interpolate(g[ep], g[ep + 1], weights, index),
+                      interpolate(b[ep], b[ep + 1], weights, index)};
+      for (int c = 0; c < 3; c++) {
+        o[c] = finish_unquantize(v[c], is_signed);
+        if (comp) comp[(i * 4 + j) * 3 + c] = v[c];
+      }
     }
 }
 

@@ -62,9 +62,9 @@ extern "C" int kcheck_bc15_encode(int fmt, const uint8_t* rgba, uint32_t width, 
 #include "../../block_compression_amd/csrc/bc6h_block.hpp"
 
 // BC6H through the same per-block function as the HIP kernel.  `rgba16` are
-// f16 bit patterns, stride in f16 elements.
+// f16 bit patterns, stride in f16 elements; is_signed selects BC6H_SF16.
 extern "C" int kcheck_bc6h_encode(const void* settings, const uint16_t* rgba16, uint32_t width,
-                                  uint32_t height, uint32_t stride, uint8_t* out) {
+                                  uint32_t height, uint32_t stride, uint8_t* out, int is_signed) {
   bcn::Bc6hParams s;
   std::memcpy(&s, settings, sizeof(s));
   uint32_t bw = width / 4, bh = height / 4;
@@ -78,7 +78,10 @@ extern "C" int kcheck_bc6h_encode(const void* settings, const uint16_t* rgba16, 
       }
     int keys[32];
     uint32_t o[4];
-    bcn::encode_bc6h_block(bits, s, keys, 1, o);
+    if (is_signed)
+      bcn::encode_bc6h_block<true>(bits, s, keys, 1, o);
+    else
+      bcn::encode_bc6h_block<false>(bits, s, keys, 1, o);
     std::memcpy(out + (size_t)idx * 16, o, 16);
   }
   return 0;

@@ -42,11 +42,12 @@ def bc15_encode(fmt: int, img: np.ndarray) -> np.ndarray:
     return out
 
 
-def bc6h_encode(settings: bytes, img16: np.ndarray) -> np.ndarray:
+def bc6h_encode(settings: bytes, img16: np.ndarray, is_signed: bool = False) -> np.ndarray:
     bits = np.ascontiguousarray(img16).view(np.uint16)
     h, w, _ = bits.shape
     out = np.zeros((w // 4) * (h // 4) * 16, np.uint8)
-    lib().kcheck_bc6h_encode(ctypes.create_string_buffer(settings), _p(bits), w, h, w * 4, _p(out))
+    lib().kcheck_bc6h_encode(ctypes.create_string_buffer(settings), _p(bits), w, h, w * 4, _p(out),
+                             int(is_signed))
     return out
 
 

@@ -44,12 +44,16 @@ def _declare(lib):
         getattr(lib, f"oracle_decode_block_{n}").restype = None
     lib.oracle_decode_block_bc6h.argtypes = [_vp, _vp, _sz, _i32]
     lib.oracle_decode_block_bc6h_float.argtypes = [_vp, _vp, _sz, _i32]
+    lib.oracle_decode_block_bc6h_comp.argtypes = [_vp, _vp, _i32]
+    lib.oracle_decode_block_bc6h_comp.restype = None
     lib.oracle_decompress_rgba8.argtypes = [_i32, _u32, _u32, _vp, _sz, _vp, _sz]
     lib.oracle_decompress_rgba8.restype = _i32
     for n in ("16f", "32f"):
         getattr(lib, f"oracle_decompress_rgba{n}").argtypes = [_i32, _u32, _u32, _vp, _sz, _vp, _sz]
         getattr(lib, f"oracle_decompress_rgba{n}").restype = _i32
     lib.oracle_linear_to_srgb.argtypes = [ctypes.c_float]
+    lib.oracle_compress_rgba16_sf16.argtypes = [_vp, _vp, _vp, _sz, _u32, _u32, _u32, _vp, _i32]
+    lib.oracle_compress_rgba16_sf16.restype = _i32
     lib.oracle_dds_header.argtypes = [_u32, _u32, _u32, _u32, _u32, _u32, _vp]
     lib.oracle_dds_header.restype = _i32
     lib.oracle_linear_to_srgb.restype = ctypes.c_uint8
@@ -118,6 +122,22 @@ def compress_rgba16(settings: bytes, rgba16: np.ndarray, threads: int = 1) -> np
     return out
 
 
+def compress_rgba16_sf16(settings: bytes, rgba16: np.ndarray, threads: int = 1):
+    """Signed (SF16) BC6H encode of a (h, w, 4) float16 image (the oracle's
+    extension of the reference's unsigned encoder); returns (blocks, the
+    search error of each block)."""
+    bits = np.ascontiguousarray(rgba16).view(np.uint16)
+    h, w, _ = bits.shape
+    out = np.zeros((w // 4) * (h // 4) * 16, np.uint8)
+    errs = np.zeros((w // 4) * (h // 4), np.float32)
+    sbuf = ctypes.create_string_buffer(settings)
+    rc = lib().oracle_compress_rgba16_sf16(sbuf, _ptr(bits), _ptr(out), out.size, w, h, w * 4,
+                                           _ptr(errs), threads)
+    if rc != 0:
+        raise ValueError(f"oracle_compress_rgba16_sf16 failed: {rc}")
+    return out, errs
+
+
 def compress_rgba8_sample(tag: int, settings: bytes, rgba: np.ndarray, block_ids: np.ndarray,
                           threads: int = 1) -> np.ndarray:
     rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
@@ -172,6 +192,14 @@ def decode_block_bc6h_float(block: bytes, pitch: int, is_signed: bool) -> np.nda
     out = np.zeros(4 * pitch, np.float32)
     lib().oracle_decode_block_bc6h_float(_ptr(src), _ptr(out), pitch, int(is_signed))
     return out
+
+
+def decode_block_bc6h_comp(block: bytes, is_signed: bool) -> np.ndarray:
+    """The 16 x 3 interpolated BC6H values before finish_unquantize."""
+    src = np.frombuffer(bytes(block), np.uint8).copy()
+    out = np.zeros(48, np.int32)
+    lib().oracle_decode_block_bc6h_comp(_ptr(src), _ptr(out), int(is_signed))
+    return out.reshape(16, 3)
 
 
 def decompress_rgba8(tag: int, width: int, height: int, blocks: np.ndarray) -> np.ndarray:

@@ -55,6 +55,8 @@ def test_dxgi_format_mapping():
     assert [dds.DXGI_NAMES[f] for f in got] == [
         "BC1_UNORM_SRGB", "BC2_UNORM_SRGB", "BC3_UNORM_SRGB", "BC4_UNORM", "BC5_UNORM",
         "BC6H_UF16", "BC7_UNORM_SRGB"]
+    # The signed BC6H extension.
+    assert dds.DXGI_NAMES[dds.dxgi_format(CompressionVariant.BC6H_SF16(BC6HSettings.basic()))] == "BC6H_SF16"
 
 
 def test_header_matches_oracle_everywhere():
