@@ -91,11 +91,12 @@ __global__ __launch_bounds__(kDecGroup) void decode_bc6h_srgb8_kernel(
   const uint32_t* src = reinterpret_cast<const uint32_t*>(blocks + (size_t)idx * 16);
   uint32_t in[4] = {src[0], src[1], src[2], src[3]};
   uint16_t h[48];
-  decode_bc6h(in, SGN, h);
+  decode_bc6h_t<SGN>(in, h);
   uint32_t yy = idx / bw, xx = idx - yy * bw;
   uint8_t* o = rgba + (size_t)yy * 4 * pitch + (size_t)xx * 16;
+  // Negative values (and -0) code as 0, the code of +0: select the input.
   auto code = [&](uint16_t v) -> uint32_t {
-    return (SGN && (v & 0x8000u)) ? 0u : srgb8_code(v, lt);
+    return srgb8_code((SGN && (v & 0x8000u)) ? (uint16_t)0 : v, lt);
   };
 #pragma unroll
   for (int y = 0; y < 4; y++) {
@@ -109,10 +110,9 @@ __global__ __launch_bounds__(kDecGroup) void decode_bc6h_srgb8_kernel(
   }
 }
 
-template <int OUT>
+template <int OUT, bool SGN>
 __global__ __launch_bounds__(kDecGroup) void decode_bc6h_kernel(const uint8_t* __restrict__ blocks,
                                                                 uint32_t bw, uint32_t nblocks,
-                                                                int is_signed,
                                                                 uint8_t* __restrict__ rgba,
                                                                 uint32_t pitch) {
   uint32_t idx = blockIdx.x * kDecGroup + threadIdx.x;
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(kDecGroup) void decode_bc6h_kernel(const uint8_t* _
   const uint32_t* src = reinterpret_cast<const uint32_t*>(blocks + (size_t)idx * 16);
   uint32_t in[4] = {src[0], src[1], src[2], src[3]};
   uint16_t h[48];
-  decode_bc6h(in, is_signed != 0, h);
+  decode_bc6h_t<SGN>(in, h);
   constexpr bool F32 = OUT == kOutF32Rgba || OUT == kOutF32RefRgb;
   uint32_t yy = idx / bw, xx = idx - yy * bw;
   uint8_t* o = rgba + (size_t)yy * 4 * pitch + (size_t)xx * (F32 ? 64 : 32);
@@ -224,12 +224,23 @@ int decode_bc6h_image(bcn_variant variant, const void* blocks, size_t blocks_siz
   uint8_t* o = static_cast<uint8_t*>(rgba);
   bool ref = flags & BCN_DECODE_REFERENCE_LAYOUT;
   int out = f32 ? (ref ? kOutF32RefRgb : kOutF32Rgba) : (ref ? kOutF16RefRgb : kOutF16Rgba);
-  switch (out) {
-    case kOutF16Rgba: hipLaunchKernelGGL(decode_bc6h_kernel<kOutF16Rgba>, grid, block, 0, s, b, bw, nblocks, is_signed, o, rgba_pitch); break;
-    case kOutF32Rgba: hipLaunchKernelGGL(decode_bc6h_kernel<kOutF32Rgba>, grid, block, 0, s, b, bw, nblocks, is_signed, o, rgba_pitch); break;
-    case kOutF16RefRgb: hipLaunchKernelGGL(decode_bc6h_kernel<kOutF16RefRgb>, grid, block, 0, s, b, bw, nblocks, is_signed, o, rgba_pitch); break;
-    default: hipLaunchKernelGGL(decode_bc6h_kernel<kOutF32RefRgb>, grid, block, 0, s, b, bw, nblocks, is_signed, o, rgba_pitch); break;
+#define BCN_DEC6(O, S) hipLaunchKernelGGL((decode_bc6h_kernel<O, S>), grid, block, 0, s, b, bw, nblocks, o, rgba_pitch)
+  if (is_signed) {
+    switch (out) {
+      case kOutF16Rgba: BCN_DEC6(kOutF16Rgba, true); break;
+      case kOutF32Rgba: BCN_DEC6(kOutF32Rgba, true); break;
+      case kOutF16RefRgb: BCN_DEC6(kOutF16RefRgb, true); break;
+      default: BCN_DEC6(kOutF32RefRgb, true); break;
+    }
+  } else {
+    switch (out) {
+      case kOutF16Rgba: BCN_DEC6(kOutF16Rgba, false); break;
+      case kOutF32Rgba: BCN_DEC6(kOutF32Rgba, false); break;
+      case kOutF16RefRgb: BCN_DEC6(kOutF16RefRgb, false); break;
+      default: BCN_DEC6(kOutF32RefRgb, false); break;
+    }
   }
+#undef BCN_DEC6
   return launch_ok() ? BCN_OK : BCN_ERR_HIP;
 }
 

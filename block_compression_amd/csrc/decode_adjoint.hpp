@@ -80,7 +80,7 @@ BCN_HD void bc7_parse(const uint32_t in[4], Bc7Form& f) {
   f.mode = (uint32_t)__builtin_ctz(first);
   const Bc7ModeInfo mi = kBc7Modes[f.mode];
   BitReader br(in);
-  br.pos = f.mode + 1;
+  br.skip((uint32_t)f.mode + 1);
   f.part = br.read(mi.pb);
   f.rot = br.read(mi.rb);
   f.isb = br.read(mi.isb);

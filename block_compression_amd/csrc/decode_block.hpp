@@ -12,20 +12,33 @@
 
 namespace bcn {
 
+// (hi:lo) >> n, low 32 bits, for n in 0..31 (v_alignbit_b32 on the GPU).
+BCN_HD uint32_t funnel_shr(uint32_t hi, uint32_t lo, uint32_t n) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_alignbit(hi, lo, n);
+#else
+  return (uint32_t)((((uint64_t)hi << 32) | lo) >> n);
+#endif
+}
+
+// LSB-first reader of a 128-bit block that consumes its bits from the bottom:
+// a read takes the low n bits (n <= 31) and shifts the four words right by n
+// with three funnel shifts.  There is no position-dependent branch, so reads
+// whose width differs per lane (mode-dependent fields, anchor indices) cost
+// the same as fixed ones, and a read of 0 bits is a no-op.
 struct BitReader {
-  uint64_t lo, hi;
-  uint32_t pos = 0;
-  BCN_HD BitReader(const uint32_t w[4])
-      : lo((uint64_t)w[0] | ((uint64_t)w[1] << 32)), hi((uint64_t)w[2] | ((uint64_t)w[3] << 32)) {}
+  uint32_t w0, w1, w2, w3;
+  BCN_HD BitReader(const uint32_t w[4]) : w0(w[0]), w1(w[1]), w2(w[2]), w3(w[3]) {}
   BCN_HD uint32_t read(uint32_t n) {
-    if (n == 0) return 0;
-    uint64_t v;
-    if (pos >= 64) v = hi >> (pos - 64);
-    else if (pos == 0) v = lo;
-    else v = (lo >> pos) | (hi << (64 - pos));
-    pos += n;
-    return (uint32_t)(v & ((1ull << n) - 1));
+    uint32_t v = w0 & ((1u << n) - 1u);
+    w0 = funnel_shr(w1, w0, n);
+    w1 = funnel_shr(w2, w1, n);
+    w2 = funnel_shr(w3, w2, n);
+    w3 >>= n;
+    return v;
   }
+  // Drop n bits (n <= 31).
+  BCN_HD void skip(uint32_t n) { (void)read(n); }
 };
 
 // (a * (64 - w) + b * w + 32) >> 6 (block.rs:1256-1258)
@@ -105,10 +118,51 @@ template <int IB>
 BCN_HD int bptc_weight(int i) {
   return IB == 2 ? weight2(i) : (IB == 3 ? weight3(i) : weight4(i));
 }
-BCN_HD int bptc_weight_rt(int bits, int i) {
-  return bits == 2 ? weight2(i) : (bits == 3 ? weight3(i) : weight4(i));
+// The BPTC index weights are round(64 i / m) with m = 2^bits - 1, and for
+// m = 3, 7, 15 they equal (i * (4095 / m) + 32) >> 6 at every index (checked
+// at compile time below against the format's tables): one multiply instead
+// of a per-lane choice among three tables when `bits` differs by lane.
+BCN_HD int bptc_weight_mul(int bits) { return bits == 2 ? 1365 : (bits == 3 ? 585 : 273); }
+BCN_HD int bptc_weight_rt(int bits, int i) { return (mul24(i, bptc_weight_mul(bits)) + 32) >> 6; }
+
+constexpr bool bptc_weight_form_holds() {
+  constexpr int t2[4] = {0, 21, 43, 64};
+  constexpr int t3[8] = {0, 9, 18, 27, 37, 46, 55, 64};
+  constexpr int t4[16] = {0, 4, 9, 13, 17, 21, 26, 30, 34, 38, 43, 47, 51, 55, 60, 64};
+  for (int i = 0; i < 4; i++)
+    if (((i * 1365 + 32) >> 6) != t2[i]) return false;
+  for (int i = 0; i < 8; i++)
+    if (((i * 585 + 32) >> 6) != t3[i]) return false;
+  for (int i = 0; i < 16; i++)
+    if (((i * 273 + 32) >> 6) != t4[i]) return false;
+  return true;
+}
+static_assert(bptc_weight_form_holds(), "BPTC weight closed form");
+
+// Two 16-bit lanes of (a * (64 - w) + b * w + 32) >> 6 (block.rs:1256-1258):
+// lo/hi/w hold one channel per half-word.  Every term stays below 2^16
+// (255 * 64 + 32), so packed 16-bit integer arithmetic is exact.
+BCN_HD uint32_t interp2(uint32_t lo, uint32_t hi, uint32_t w) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+  const u16x2 a = __builtin_bit_cast(u16x2, lo), b = __builtin_bit_cast(u16x2, hi);
+  const u16x2 ww = __builtin_bit_cast(u16x2, w);
+  const u16x2 c64 = {64, 64}, c32 = {32, 32}, c6 = {6, 6};
+  const u16x2 r = (a * (c64 - ww) + b * ww + c32) >> c6;
+  return __builtin_bit_cast(uint32_t, r);
+#else
+  uint32_t r = 0;
+  for (int h = 0; h < 2; h++) {
+    uint32_t a = (lo >> (16 * h)) & 0xFFFFu, b = (hi >> (16 * h)) & 0xFFFFu;
+    uint32_t ww = (w >> (16 * h)) & 0xFFFFu;
+    r |= (((a * (64 - ww) + b * ww + 32) >> 6) & 0xFFFFu) << (16 * h);
+  }
+  return r;
+#endif
 }
 
+// Every read below has a per-lane width (0 where the mode lacks the field)
+// rather than a per-lane branch, so a wave of mixed modes runs one path.
 BCN_HD void decode_bc7(const uint32_t in[4], uint32_t px[16]) {
   uint32_t first = in[0] & 0xFFu;
   if (first == 0) {
@@ -119,86 +173,85 @@ BCN_HD void decode_bc7(const uint32_t in[4], uint32_t px[16]) {
   int mode = __builtin_ctz(first);
   const Bc7ModeInfo mi = kBc7Modes[mode];
   BitReader br(in);
-  br.pos = (uint32_t)mode + 1;
+  br.skip((uint32_t)mode + 1);
   int part = (int)br.read(mi.pb);
   int rotation = (int)br.read(mi.rb);
   int isb = (int)br.read(mi.isb);
   const int ne = mi.ns * 2;
-  // Endpoint arrays are indexed with compile-time indices only (unrolled,
-  // guarded loops and selects) so they stay in registers.
+  // Endpoint arrays are indexed with compile-time indices only (unrolled
+  // loops and selects) so they stay in registers.
   int ep[6][4];
 #pragma unroll
   for (int c = 0; c < 3; c++)
 #pragma unroll
-    for (int e = 0; e < 6; e++) ep[e][c] = e < ne ? (int)br.read(mi.cb) : 0;
+    for (int e = 0; e < 6; e++) ep[e][c] = (int)br.read(e < ne ? mi.cb : 0u);
 #pragma unroll
-  for (int e = 0; e < 6; e++) ep[e][3] = (e < ne && mi.ab) ? (int)br.read(mi.ab) : 0;
-  int cbits = mi.cb, abits = mi.ab;
-  if (mi.pbit == 1) {
+  for (int e = 0; e < 6; e++) ep[e][3] = (int)br.read(e < ne ? mi.ab : 0u);
+  // P-bits: one per endpoint (kind 1) or one per subset shared by its two
+  // endpoints' colour channels (kind 2); at most one of the two loops reads.
+  int pb[6];
 #pragma unroll
-    for (int e = 0; e < 6; e++) {
-      if (e < ne) {
-        int b = (int)br.read(1);
+  for (int e = 0; e < 6; e++) pb[e] = (int)br.read(mi.pbit == 1 && e < ne ? 1u : 0u);
 #pragma unroll
-        for (int c = 0; c < 4; c++) ep[e][c] = (ep[e][c] << 1) | b;
-      }
-    }
-    cbits++;
-    abits++;
-  } else if (mi.pbit == 2) {
-#pragma unroll
-    for (int sb = 0; sb < 2; sb++) {
-      int b = (int)br.read(1);
-#pragma unroll
-      for (int c = 0; c < 3; c++) {
-        ep[2 * sb][c] = (ep[2 * sb][c] << 1) | b;
-        ep[2 * sb + 1][c] = (ep[2 * sb + 1][c] << 1) | b;
-      }
-    }
-    cbits++;
+  for (int sb = 0; sb < 2; sb++) {
+    int b = (int)br.read(mi.pbit == 2 ? 1u : 0u);
+    pb[2 * sb] |= b;
+    pb[2 * sb + 1] |= b;
   }
+  const int has_p = mi.pbit != 0, alpha_p = mi.pbit == 1;
+  const int cbits = mi.cb + has_p, abits = mi.ab + alpha_p;
+  // Expand, apply the rotation to the endpoints (output channel rotation-1
+  // takes the alpha endpoints and alpha the channel's, block.rs:1228-1240),
+  // and pack each subset's endpoints as (R | G << 16), (B | A << 16).
+  uint32_t lrg[3], lba[3], hrg[3], hba[3];
 #pragma unroll
-  for (int e = 0; e < 6; e++) {
+  for (int sb = 0; sb < 3; sb++) {
+    int v[2][4];
 #pragma unroll
-    for (int c = 0; c < 3; c++) ep[e][c] = expand_bits(ep[e][c], cbits);
-    ep[e][3] = mi.ab ? expand_bits(ep[e][3], abits) : 255;
+    for (int h = 0; h < 2; h++) {
+      const int e = 2 * sb + h;
+      int c[4];
+#pragma unroll
+      for (int ch = 0; ch < 3; ch++) c[ch] = expand_bits(has_p ? (ep[e][ch] << 1) | pb[e] : ep[e][ch], cbits);
+      c[3] = mi.ab ? expand_bits(alpha_p ? (ep[e][3] << 1) | pb[e] : ep[e][3], abits) : 255;
+      v[h][0] = rotation == 1 ? c[3] : c[0];
+      v[h][1] = rotation == 2 ? c[3] : c[1];
+      v[h][2] = rotation == 3 ? c[3] : c[2];
+      v[h][3] = rotation == 1 ? c[0] : (rotation == 2 ? c[1] : (rotation == 3 ? c[2] : c[3]));
+    }
+    lrg[sb] = (uint32_t)v[0][0] | ((uint32_t)v[0][1] << 16);
+    lba[sb] = (uint32_t)v[0][2] | ((uint32_t)v[0][3] << 16);
+    hrg[sb] = (uint32_t)v[1][0] | ((uint32_t)v[1][1] << 16);
+    hba[sb] = (uint32_t)v[1][2] | ((uint32_t)v[1][3] << 16);
   }
+  // Half-words interpolated with the alpha-set weight: the output channel
+  // that holds the alpha endpoints.
+  const uint32_t mrg = (rotation == 1 ? 0x0000FFFFu : 0u) | (rotation == 2 ? 0xFFFF0000u : 0u);
+  const uint32_t mba = (rotation == 3 ? 0x0000FFFFu : 0u) | (rotation == 0 ? 0xFFFF0000u : 0u);
   uint32_t pattern = mi.ns == 1 ? 0u : kPart.subset[mi.ns == 3 ? 64 + part : part];
   int a1 = mi.ns >= 2 ? kPart.anchor1[mi.ns == 3 ? 64 + part : part] : -1;
   int a2 = mi.ns == 3 ? kPart.anchor2[64 + part] : -1;
-  int idx[16];
+  const int k1 = bptc_weight_mul(mi.ib), k2 = bptc_weight_mul(mi.ib2);
+  int w1[16];
 #pragma unroll
   for (int k = 0; k < 16; k++) {
     bool anchor = k == 0 || k == a1 || k == a2;
-    idx[k] = (int)br.read(anchor ? mi.ib - 1 : mi.ib);
+    w1[k] = (mul24((int)br.read(anchor ? mi.ib - 1u : mi.ib), k1) + 32) >> 6;
   }
 #pragma unroll
   for (int k = 0; k < 16; k++) {
     int s = (int)((pattern >> (2 * k)) & 3u);
-    int w = bptc_weight_rt(mi.ib, idx[k]);
-    int lo[4], hi[4];
-#pragma unroll
-    for (int ch = 0; ch < 4; ch++) {
-      lo[ch] = s == 0 ? ep[0][ch] : (s == 1 ? ep[2][ch] : ep[4][ch]);
-      hi[ch] = s == 0 ? ep[1][ch] : (s == 1 ? ep[3][ch] : ep[5][ch]);
-    }
-    int c[4];
-    if (mi.ib2 == 0) {
-#pragma unroll
-      for (int ch = 0; ch < 4; ch++) c[ch] = interp(lo[ch], hi[ch], w);
-    } else {
-      int i2 = (int)br.read(k == 0 ? mi.ib2 - 1 : mi.ib2);
-      int w2 = bptc_weight_rt(mi.ib2, i2);
-      int wc = isb ? w2 : w, wa = isb ? w : w2;
-#pragma unroll
-      for (int ch = 0; ch < 3; ch++) c[ch] = interp(lo[ch], hi[ch], wc);
-      c[3] = interp(lo[3], hi[3], wa);
-    }
-    int r0 = rotation == 1 ? c[3] : c[0];
-    int g0 = rotation == 2 ? c[3] : c[1];
-    int b0 = rotation == 3 ? c[3] : c[2];
-    int a0 = rotation == 1 ? c[0] : (rotation == 2 ? c[1] : (rotation == 3 ? c[2] : c[3]));
-    px[k] = (uint32_t)r0 | ((uint32_t)g0 << 8) | ((uint32_t)b0 << 16) | ((uint32_t)a0 << 24);
+    // Second index set (modes 4 and 5), read after all of the first.
+    int i2 = (int)br.read(mi.ib2 == 0 ? 0u : (k == 0 ? mi.ib2 - 1u : mi.ib2));
+    int w2 = mi.ib2 == 0 ? w1[k] : (mul24(i2, k2) + 32) >> 6;
+    uint32_t wc = (uint32_t)(isb ? w2 : w1[k]), wa = (uint32_t)(isb ? w1[k] : w2);
+    wc |= wc << 16;
+    wa |= wa << 16;
+    uint32_t rg = interp2(s == 0 ? lrg[0] : (s == 1 ? lrg[1] : lrg[2]),
+                          s == 0 ? hrg[0] : (s == 1 ? hrg[1] : hrg[2]), (wc & ~mrg) | (wa & mrg));
+    uint32_t ba = interp2(s == 0 ? lba[0] : (s == 1 ? lba[1] : lba[2]),
+                          s == 0 ? hba[0] : (s == 1 ? hba[1] : hba[2]), (wc & ~mba) | (wa & mba));
+    px[k] = (rg & 0xFFu) | ((rg >> 8) & 0xFF00u) | ((ba & 0xFFu) << 16) | ((ba >> 16) << 24);
   }
 }
 
@@ -206,37 +259,54 @@ BCN_HD void decode_bc7(const uint32_t in[4], uint32_t px[16]) {
 
 BCN_HD int extend_sign(int v, int bits) { return (int)((uint32_t)v << (32 - bits)) >> (32 - bits); }
 
-BCN_HD int bc6h_unquantize(int v, int bits, bool is_signed) {
-  if (!is_signed) {
-    if (bits >= 15) return v;
-    if (v == 0) return 0;
-    if (v == (1 << bits) - 1) return 0xFFFF;
-    return ((v << 16) + 0x8000) >> bits;
+// unquantize (block.rs:1369-1389), written as selects.
+template <bool SIGNED>
+BCN_HD int bc6h_unquantize(int v, int bits) {
+  if (!SIGNED) {
+    int u = ((v << 16) + 0x8000) >> bits;
+    u = v == (1 << bits) - 1 ? 0xFFFF : u;
+    u = v == 0 ? 0 : u;
+    return bits >= 15 ? v : u;
   }
-  if (bits >= 16) return v;
-  bool s = v < 0;
-  int m = s ? -v : v;
-  int u;
-  if (m == 0) u = 0;
-  else if (m >= (1 << (bits - 1)) - 1) u = 0x7FFF;
-  else u = ((m << 15) + 0x4000) >> (bits - 1);
-  return s ? -u : u;
+  int m = v < 0 ? -v : v;
+  int u = ((m << 15) + 0x4000) >> (bits - 1);
+  u = m >= (1 << (bits - 1)) - 1 ? 0x7FFF : u;
+  u = m == 0 ? 0 : u;
+  u = v < 0 ? -u : u;
+  return bits >= 16 ? v : u;
 }
 
-BCN_HD uint16_t bc6h_finish(int v, bool is_signed) {
-  if (!is_signed) return (uint16_t)((v * 31) >> 6);
-  int sc = v < 0 ? -(((-v) * 31) >> 5) : (v * 31) >> 5;
-  return sc < 0 ? (uint16_t)(0x8000 | (uint32_t)(-sc)) : (uint16_t)sc;
+// finish_unquantize (block.rs:1391-1397): the half bit pattern.  A negative
+// value whose magnitude scales to 0 gives +0.
+template <bool SIGNED>
+BCN_HD uint16_t bc6h_finish(int v) {
+  if (!SIGNED) return (uint16_t)((v * 31) >> 6);
+  uint32_t mag = (uint32_t)(((v < 0 ? -v : v) * 31) >> 5);
+  return (uint16_t)((v < 0 && mag != 0) ? (0x8000u | mag) : mag);
 }
 
 // Base-endpoint bits per mode (the decoder's W precision).
 BCN_TABLE uint8_t kBc6hBaseBits[14] = {10, 7, 11, 11, 11, 9, 8, 8, 8, 6, 10, 11, 12, 16};
 
-// RGB half-float bit patterns of the 16 texels: out[k*3 + c].
-BCN_HD void decode_bc6h(const uint32_t in[4], bool is_signed, uint16_t out[48]) {
+// The low n bits of v in reverse order (n in 1..31).
+BCN_HD uint32_t reverse_low_bits(uint32_t v, uint32_t n) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_bitreverse32(v) >> (32u - n);
+#else
+  uint32_t r = 0;
+  for (uint32_t k = 0; k < n; k++) r |= ((v >> k) & 1u) << (n - 1 - k);
+  return r;
+#endif
+}
+
+// RGB half-float bit patterns of the 16 texels: out[k*3 + c].  As in
+// decode_bc7, per-lane differences (mode, field widths) are read widths and
+// selects, not branches.
+template <bool SIGNED>
+BCN_HD void decode_bc6h_t(const uint32_t in[4], uint16_t out[48]) {
   BitReader br(in);
   uint32_t mb = br.read(2);
-  if (mb > 1) mb |= br.read(3) << 2;
+  mb |= br.read(mb > 1 ? 3u : 0u) << 2;
   int m = -1;
 #pragma unroll
   for (int i = 0; i < 14; i++) m = (kBc6hPrefix[i] == mb) ? i : m;
@@ -244,65 +314,76 @@ BCN_HD void decode_bc6h(const uint32_t in[4], bool is_signed, uint16_t out[48]) 
     for (int i = 0; i < 48; i++) out[i] = 0;  // reserved mode: RGB zero
     return;
   }
-  // Field value of (endpoint e, channel c) lives in f[e*3 + c]; fields come
-  // with runtime (e, c), so they are merged through selects, keeping f in
-  // registers.
+  // Field value of (endpoint e, channel c) goes to slot e*3 + c; slots are
+  // kept as 16-bit halves of six words (every field value is below 2^16
+  // before sign extension) and merged through selects, so they stay in
+  // registers although the slot differs by lane.  Every mode runs the
+  // longest field list: the table's entries past a mode's count are 0, a
+  // read of 0 bits into slot 0.
+  uint32_t f2[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int i = 0; i < 23; i++) {
+    uint32_t t = kBc6hFields[m][i];
+    uint32_t c = t & 3u, e = (t >> 2) & 3u, first = (t >> 4) & 31u, n = (t >> 9) & 63u;
+    uint32_t raw = br.read(n);
+    // A reversed field stores its bits from first+n-1 down to first.
+    uint32_t v = ((t >> 15) ? reverse_low_bits(raw, n) : raw) << first;
+    uint32_t slot = e * 3 + c;
+    v <<= (slot & 1u) * 16;
+#pragma unroll
+    for (uint32_t w = 0; w < 6; w++) f2[w] |= w == (slot >> 1) ? v : 0u;
+  }
   int f[12];
 #pragma unroll
-  for (int i = 0; i < 12; i++) f[i] = 0;
-  for (int i = 0; i < kBc6hFieldCount[m]; i++) {
-    uint32_t t = kBc6hFields[m][i];
-    int c = (int)(t & 3u), e = (int)((t >> 2) & 3u), first = (int)((t >> 4) & 31u);
-    int n = (int)((t >> 9) & 63u);
-    int v;
-    if (t >> 15) {
-      v = 0;
-      for (int k = 0; k < n; k++) v |= (int)br.read(1) << (first + n - 1 - k);
-    } else {
-      v = (int)br.read((uint32_t)n) << first;
-    }
-    int slot = e * 3 + c;
-#pragma unroll
-    for (int j = 0; j < 12; j++) f[j] |= j == slot ? v : 0;
-  }
+  for (int i = 0; i < 12; i++) f[i] = (int)((f2[i >> 1] >> (16 * (i & 1))) & 0xFFFFu);
   const bool two = m < 10;
-  int part = two ? (int)br.read(5) : 0;
+  int part = (int)br.read(two ? 5u : 0u);
   const int ne = two ? 4 : 2;
   const int b0 = kBc6hBaseBits[m];
-  if (is_signed)
+  const bool transformed = m != 9 && m != 10;
+  if (SIGNED)
 #pragma unroll
     for (int c = 0; c < 3; c++) f[c] = extend_sign(f[c], b0);
-  if ((m != 9 && m != 10) || is_signed)
 #pragma unroll
-    for (int e = 1; e < 4; e++)
+  for (int e = 1; e < 4; e++)
 #pragma unroll
-      for (int c = 0; c < 3; c++)
-        if (e < ne) f[e * 3 + c] = extend_sign(f[e * 3 + c], kBc6hDeltaBits[c][m]);
-  if (m != 9 && m != 10)
-#pragma unroll
-    for (int e = 1; e < 4; e++)
-#pragma unroll
-      for (int c = 0; c < 3; c++) {
-        int t = (f[e * 3 + c] + f[c]) & ((1 << b0) - 1);
-        if (e < ne) f[e * 3 + c] = is_signed ? extend_sign(t, b0) : t;
+    for (int c = 0; c < 3; c++) {
+      int v = f[e * 3 + c];
+      if (SIGNED) {
+        v = extend_sign(v, kBc6hDeltaBits[c][m]);
+      } else {
+        v = transformed ? extend_sign(v, kBc6hDeltaBits[c][m]) : v;
       }
+      int t = (v + f[c]) & ((1 << b0) - 1);
+      v = transformed ? (SIGNED ? extend_sign(t, b0) : t) : v;
+      f[e * 3 + c] = e < ne ? v : f[e * 3 + c];
+    }
 #pragma unroll
-  for (int i = 0; i < 12; i++) f[i] = bc6h_unquantize(f[i], b0, is_signed);
+  for (int i = 0; i < 12; i++) f[i] = bc6h_unquantize<SIGNED>(f[i], b0);
   uint32_t pattern = two ? kPart.subset[part] : 0u;
   int a1 = two ? kPart.anchor1[part] : -1;
   const int ib = two ? 3 : 4;
+  const int kw = bptc_weight_mul(ib);
 #pragma unroll
   for (int k = 0; k < 16; k++) {
     bool anchor = k == 0 || k == a1;
-    int i = (int)br.read(anchor ? ib - 1 : ib);
+    int w = (mul24i((int)br.read(anchor ? ib - 1 : ib), kw) + 32) >> 6;
     int s = (int)((pattern >> (2 * k)) & 1u);
-    int w = ib == 3 ? weight3(i) : weight4(i);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       int lo = s ? f[6 + c] : f[c], hi = s ? f[9 + c] : f[3 + c];
-      out[k * 3 + c] = bc6h_finish(interp(lo, hi, w), is_signed);
+      // (lo*(64-w) + hi*w + 32) >> 6 == lo + ((w*(hi-lo) + 32) >> 6): 64*lo is
+      // a multiple of 64 and >> floors; |w*(hi-lo)| <= 64 * 65535 < 2^23.
+      out[k * 3 + c] = bc6h_finish<SIGNED>(lo + ((mul24i(w, hi - lo) + 32) >> 6));
     }
   }
+}
+
+BCN_HD void decode_bc6h(const uint32_t in[4], bool is_signed, uint16_t out[48]) {
+  if (is_signed)
+    decode_bc6h_t<true>(in, out);
+  else
+    decode_bc6h_t<false>(in, out);
 }
 
 #if defined(__HIPCC__)
@@ -322,21 +403,21 @@ __device__ __forceinline__ float half_to_float(uint16_t h) {
 // within one code of linear_to_srgb on every input.
 __device__ __forceinline__ int srgb8_estimate(uint16_t h) {
   float lin = half_to_float(h);
-  float v = lin <= 0.0031308f
-                ? lin * 12.92f
-                : 1.055f * __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(lin) * (1.0f / 2.4f)) - 0.055f;
+  // Both branches are computed and selected: no per-lane branch.
+  float curve = 1.055f * __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(lin) * (1.0f / 2.4f)) - 0.055f;
+  float v = lin <= 0.0031308f ? lin * 12.92f : curve;
   return (int)(fminf(fmaxf(v, 0.0f), 1.0f) * 255.0f + 0.5f);
 }
 
 // linear_to_srgb of an unsigned BC6H texel value, exactly: the estimate is
-// corrected by one comparison against each neighbouring threshold.
+// corrected by one comparison against each neighbouring threshold (both
+// thresholds are read and the correction is a select).
 __device__ __forceinline__ uint32_t srgb8_code(uint16_t h, const uint16_t* lt) {
   int k = srgb8_estimate(h);
-  if (k > 0 && h < lt[k])
-    k--;
-  else if (k < 255 && h >= lt[k + 1])
-    k++;
-  return (uint32_t)k;
+  const uint32_t below = lt[k], above = lt[k < 255 ? k + 1 : 255];
+  // Bitwise, not short-circuit, conditions: both reads stay unconditional.
+  const int dn = (int)((k > 0) & (h < below)), up = (int)((k < 255) & (h >= above));
+  return (uint32_t)(k - dn + (up & (dn ^ 1)));
 }
 #endif
 
