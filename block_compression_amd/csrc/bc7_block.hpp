@@ -346,57 +346,58 @@ template <int CH, int BITS, int NSUB, int ROT = 0>
 BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pattern,
                        uint32_t qblock[2], uint32_t sh = 0) {
   constexpr int L = 1 << BITS;
-  int dd[3][4] = {};
+  // Per subset: endpoint 0 and the endpoint difference as (R, B) and (G, A)
+  // pairs of 16-bit lanes (A lanes 0 for 3 channels).  Lane values: endpoints
+  // 0..255, differences and endpoint-minus-texel in [-255, 255],
+  // (w*diff + 32) in [-16288, 16352], interpolated minus texel in
+  // [-510, 510]: all exact in 16 bits, squares and products in the 32-bit
+  // dot products.
+  uint32_t ea_rb[3] = {0, 0, 0}, ea_ga[3] = {0, 0, 0}, de_rb[3] = {0, 0, 0}, de_ga[3] = {0, 0, 0};
   float fdiv[3] = {1.0f, 1.0f, 1.0f}, frcp[3] = {1.0f, 1.0f, 1.0f};
 #pragma unroll
   for (int j = 0; j < NSUB; j++) {
-    int div = 0;
-#pragma unroll
-    for (int p = 0; p < CH; p++) {
-      dd[j][p] = d[j][1][p] - d[j][0][p];
-      div += mul24(dd[j][p], dd[j][p]);
-    }
+    ea_rb[j] = pk16(d[j][0][0], d[j][0][2]);
+    ea_ga[j] = pk16(d[j][0][1], CH == 4 ? d[j][0][3] : 0);
+    de_rb[j] = pk16(d[j][1][0] - d[j][0][0], d[j][1][2] - d[j][0][2]);
+    de_ga[j] = pk16(d[j][1][1] - d[j][0][1], CH == 4 ? d[j][1][3] - d[j][0][3] : 0);
+    int div = dot2_16(de_ga[j], de_ga[j], dot2_16(de_rb[j], de_rb[j], 0));
     fdiv[j] = (float)div;
     // div == 0 gives +inf here and NaN below (proj is then 0), as 0 / 0 does.
     // div in [1, 260100], or 0: then rcp_normal gives NaN instead of +inf,
     // and proj is 0, so the quotient below is NaN either way.
     frcp[j] = rcp_normal(fdiv[j]);
   }
+  const uint32_t ga_mask = CH == 4 ? 0x00FF00FFu : 0x000000FFu;
   int total = 0;
   qblock[0] = 0;
   qblock[1] = 0;
 #pragma unroll
   for (int k = 0; k < 16; k++) {
     int j = NSUB == 1 ? 0 : (int)((pattern >> (2 * k)) & 3u);
-    int ea[4], de[4];
-    float div, rdiv;
-#pragma unroll
-    for (int p = 0; p < CH; p++) {
-      ea[p] = NSUB == 1 ? d[0][0][p] : (j == 0 ? d[0][0][p] : (j == 1 ? d[1][0][p] : d[2][0][p]));
-      de[p] = NSUB == 1 ? dd[0][p] : (j == 0 ? dd[0][p] : (j == 1 ? dd[1][p] : dd[2][p]));
-    }
-    div = NSUB == 1 ? fdiv[0] : (j == 0 ? fdiv[0] : (j == 1 ? fdiv[1] : fdiv[2]));
-    rdiv = NSUB == 1 ? frcp[0] : (j == 0 ? frcp[0] : (j == 1 ? frcp[1] : frcp[2]));
+    // Subset selects with constant indices keep the arrays in registers.
+#define BCN_PICK(v) (NSUB == 1 ? v[0] : (j == 0 ? v[0] : (j == 1 ? v[1] : v[2])))
+    const uint32_t erb = BCN_PICK(ea_rb), ega = BCN_PICK(ea_ga);
+    const uint32_t drb = BCN_PICK(de_rb), dga = BCN_PICK(de_ga);
+#undef BCN_PICK
+    float div = NSUB == 1 ? fdiv[0] : (j == 0 ? fdiv[0] : (j == 1 ? fdiv[1] : fdiv[2]));
+    float rdiv = NSUB == 1 ? frcp[0] : (j == 0 ? frcp[0] : (j == 1 ? frcp[1] : frcp[2]));
     uint32_t t = texel<ROT>(px, k, sh);
-    int proj = 0;
-#pragma unroll
-    for (int p = 0; p < CH; p++) proj += mul24(chi(t, p) - ea[p], de[p]);
+    // base = endpoint 0 - texel, per channel.
+    const uint32_t brb = pk_sub16(erb, t & 0x00FF00FFu), bga = pk_sub16(ega, (t >> 8) & ga_mask);
+    // proj = sum (texel - e0) * (e1 - e0) = -sum base * diff.
+    int proj = -dot2_16(bga, dga, dot2_16(brb, drb, 0));
     // |proj| <= |texel - e0| |e1 - e0| <= 510 sqrt(div) (Cauchy-Schwarz).
     float fp = div_rcp((float)proj, div, rdiv);
     int q1 = qclamp(fp * (float)L + 0.5f, 1.0f, (float)(L - 1));
-    int w0 = weight<BITS>(q1 - 1), w1 = weight<BITS>(q1);
-    int err0 = 0, err1 = 0;
-#pragma unroll
-    for (int p = 0; p < CH; p++) {
-      // The reference's ((64 - w) * a + w * b + 32) >> 6 with b = a + de: the
-      // 64 * a term is a multiple of 64 and >> is floor division, so it equals
-      // a + ((w * de + 32) >> 6).
-      int base = ea[p] - chi(t, p);
-      int t0 = base + ((mul24(w0, de[p]) + 32) >> 6);
-      int t1 = base + ((mul24(w1, de[p]) + 32) >> 6);
-      err0 += mul24(t0, t0);
-      err1 += mul24(t1, t1);
-    }
+    const uint32_t w0 = (uint32_t)weight<BITS>(q1 - 1) * 0x10001u;
+    const uint32_t w1 = (uint32_t)weight<BITS>(q1) * 0x10001u;
+    // The reference's ((64 - w) * a + w * b + 32) >> 6 with b = a + diff: the
+    // 64 * a term is a multiple of 64 and >> is floor division, so it equals
+    // a + ((w * diff + 32) >> 6); minus the texel, base + that.
+    const uint32_t t0rb = pk_add16(brb, pk_wdelta16(w0, drb)), t0ga = pk_add16(bga, pk_wdelta16(w0, dga));
+    const uint32_t t1rb = pk_add16(brb, pk_wdelta16(w1, drb)), t1ga = pk_add16(bga, pk_wdelta16(w1, dga));
+    int err0 = dot2_16(t0ga, t0ga, dot2_16(t0rb, t0rb, 0));
+    int err1 = dot2_16(t1ga, t1ga, dot2_16(t1rb, t1rb, 0));
     bool lo = err0 < err1;
     int q = lo ? q1 - 1 : q1;
     total += lo ? err0 : err1;

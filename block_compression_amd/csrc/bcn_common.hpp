@@ -67,6 +67,59 @@ BCN_HD int mul24i(int a, int b) {
 #endif
 }
 
+// ---- two signed 16-bit lanes per word ---------------------------------------
+//
+// The integer-exact parts of the 8-bit encoders work on channel pairs: an
+// RGBA8 texel word gives (R, B) = t & 0x00FF00FF and (G, A) = (t >> 8) &
+// 0x00FF00FF.  Callers keep every lane value within [-2^15, 2^15), so the
+// packed 16-bit integer instructions (v_pk_*_i16/u16, v_dot2_i32_i16) give the
+// same integers as 32-bit arithmetic.  Host builds compute lane by lane.
+
+BCN_HD uint32_t pk16(int lo, int hi) { return ((uint32_t)lo & 0xFFFFu) | ((uint32_t)hi << 16); }
+
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef short bcn_s16x2 __attribute__((ext_vector_type(2)));
+BCN_HD bcn_s16x2 as_s16x2(uint32_t v) { return __builtin_bit_cast(bcn_s16x2, v); }
+BCN_HD uint32_t as_u32(bcn_s16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+#else
+BCN_HD int lane16(uint32_t v, int h) { return (int)(int16_t)(uint16_t)(v >> (16 * h)); }
+#endif
+
+// Lane-wise a - b and a + b.
+BCN_HD uint32_t pk_sub16(uint32_t a, uint32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return as_u32(as_s16x2(a) - as_s16x2(b));
+#else
+  return pk16(lane16(a, 0) - lane16(b, 0), lane16(a, 1) - lane16(b, 1));
+#endif
+}
+BCN_HD uint32_t pk_add16(uint32_t a, uint32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return as_u32(as_s16x2(a) + as_s16x2(b));
+#else
+  return pk16(lane16(a, 0) + lane16(b, 0), lane16(a, 1) + lane16(b, 1));
+#endif
+}
+
+// Lane-wise (w * d + 32) >> 6 (arithmetic shift), |w * d + 32| < 2^15.
+BCN_HD uint32_t pk_wdelta16(uint32_t w, uint32_t d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const bcn_s16x2 c32 = {32, 32}, c6 = {6, 6};
+  return as_u32((as_s16x2(w) * as_s16x2(d) + c32) >> c6);
+#else
+  return pk16((lane16(w, 0) * lane16(d, 0) + 32) >> 6, (lane16(w, 1) * lane16(d, 1) + 32) >> 6);
+#endif
+}
+
+// a.lo * b.lo + a.hi * b.hi + c with signed lanes and 32-bit products.
+BCN_HD int dot2_16(uint32_t a, uint32_t b, int c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_sdot2(as_s16x2(a), as_s16x2(b), c, false);
+#else
+  return lane16(a, 0) * lane16(b, 0) + lane16(a, 1) * lane16(b, 1) + c;
+#endif
+}
+
 // a / b from y = 1.0f / b (Markstein's correction step): q = a*y, then the
 // exact residual r = a - b*q by FMA, then q + r*y rounded once.  This is not
 // correctly rounded for every (a, b), so it is used only at call sites whose
