@@ -110,33 +110,49 @@ struct Moments {
   int n, s[4], q[10];
 };
 
+// Four texels at a time: the masked texel words are transposed to one word
+// per channel (byte i = texel i of the group, two v_perm_b32 per channel),
+// and every sum of products over the group is one v_dot4_u32_u8.
 template <int CH, int ROT = 0>
 BCN_HD Moments moments(const uint32_t px[16], uint32_t mask, uint32_t sh = 0) {
   Moments m{};
+  m.n = __builtin_popcount(mask & 0xFFFFu);
+  uint32_t s[4] = {0, 0, 0, 0}, q[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-  for (int k = 0; k < 16; k++) {
-    int f = (int)((mask >> k) & 1u);
-    uint32_t t = texel<ROT>(px, k, sh);
-    int v0 = f ? chi(t, 0) : 0, v1 = f ? chi(t, 1) : 0, v2 = f ? chi(t, 2) : 0;
-    m.n += f;
-    m.s[0] += v0;
-    m.s[1] += v1;
-    m.s[2] += v2;
-    m.q[0] += v0 * v0;
-    m.q[1] += v0 * v1;
-    m.q[2] += v0 * v2;
-    m.q[4] += v1 * v1;
-    m.q[5] += v1 * v2;
-    m.q[7] += v2 * v2;
+  for (int g = 0; g < 4; g++) {
+    uint32_t t[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int k = 4 * g + i;
+      t[i] = ((mask >> k) & 1u) ? texel<ROT>(px, k, sh) : 0u;
+    }
+    // (t0.R, t1.R, t0.B, t1.B), (t0.G, t1.G, t0.A, t1.A), then with t2, t3.
+    const uint32_t a = perm_b32(t[1], t[0], 0x06020400u), b = perm_b32(t[1], t[0], 0x07030501u);
+    const uint32_t c = perm_b32(t[3], t[2], 0x06020400u), d = perm_b32(t[3], t[2], 0x07030501u);
+    const uint32_t r = perm_b32(c, a, 0x05040100u), bl = perm_b32(c, a, 0x07060302u);
+    const uint32_t gr = perm_b32(d, b, 0x05040100u);
+    s[0] = udot4(r, 0x01010101u, s[0]);
+    s[1] = udot4(gr, 0x01010101u, s[1]);
+    s[2] = udot4(bl, 0x01010101u, s[2]);
+    q[0] = udot4(r, r, q[0]);
+    q[1] = udot4(r, gr, q[1]);
+    q[2] = udot4(r, bl, q[2]);
+    q[4] = udot4(gr, gr, q[4]);
+    q[5] = udot4(gr, bl, q[5]);
+    q[7] = udot4(bl, bl, q[7]);
     if (CH == 4) {
-      int v3 = f ? chi(t, 3) : 0;
-      m.s[3] += v3;
-      m.q[3] += v0 * v3;
-      m.q[6] += v1 * v3;
-      m.q[8] += v2 * v3;
-      m.q[9] += v3 * v3;
+      const uint32_t al = perm_b32(d, b, 0x07060302u);
+      s[3] = udot4(al, 0x01010101u, s[3]);
+      q[3] = udot4(r, al, q[3]);
+      q[6] = udot4(gr, al, q[6]);
+      q[8] = udot4(bl, al, q[8]);
+      q[9] = udot4(al, al, q[9]);
     }
   }
+#pragma unroll
+  for (int p = 0; p < 4; p++) m.s[p] = (int)s[p];
+#pragma unroll
+  for (int i = 0; i < 10; i++) m.q[i] = (int)q[i];
   return m;
 }
 

@@ -111,6 +111,33 @@ BCN_HD uint32_t pk_wdelta16(uint32_t w, uint32_t d) {
 #endif
 }
 
+// Byte permute (v_perm_b32): byte i of the result is byte sel.byte_i of
+// {src1 (0..3), src0 (4..7)}; selector 12 gives 0.
+BCN_HD uint32_t perm_b32(uint32_t src0, uint32_t src1, uint32_t sel) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_perm(src0, src1, sel);
+#else
+  uint64_t bytes = ((uint64_t)src0 << 32) | src1;
+  uint32_t r = 0;
+  for (int i = 0; i < 4; i++) {
+    uint32_t s = (sel >> (8 * i)) & 0xFFu;
+    uint32_t b = s < 8 ? (uint32_t)(bytes >> (8 * s)) & 0xFFu : 0u;
+    r |= b << (8 * i);
+  }
+  return r;
+#endif
+}
+
+// Sum of the four byte products of a and b, plus c (v_dot4_u32_u8).
+BCN_HD uint32_t udot4(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_udot4(a, b, c, false);
+#else
+  for (int i = 0; i < 4; i++) c += ((a >> (8 * i)) & 0xFFu) * ((b >> (8 * i)) & 0xFFu);
+  return c;
+#endif
+}
+
 // a.lo * b.lo + a.hi * b.hi + c with signed lanes and 32-bit products.
 BCN_HD int dot2_16(uint32_t a, uint32_t b, int c) {
 #if defined(__HIP_DEVICE_COMPILE__)
