@@ -405,8 +405,8 @@ BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pat
     // |proj| <= |texel - e0| |e1 - e0| <= 510 sqrt(div) (Cauchy-Schwarz).
     float fp = div_rcp((float)proj, div, rdiv);
     int q1 = qclamp(fp * (float)L + 0.5f, 1.0f, (float)(L - 1));
-    const uint32_t w0 = (uint32_t)weight<BITS>(q1 - 1) * 0x10001u;
-    const uint32_t w1 = (uint32_t)weight<BITS>(q1) * 0x10001u;
+    const uint32_t w0 = (uint32_t)weight_mad<BITS>(q1 - 1) * 0x10001u;
+    const uint32_t w1 = (uint32_t)weight_mad<BITS>(q1) * 0x10001u;
     // The reference's ((64 - w) * a + w * b + 32) >> 6 with b = a + diff: the
     // 64 * a term is a multiple of 64 and >> is floor division, so it equals
     // a + ((w * diff + 32) >> 6); minus the texel, base + that.
@@ -430,23 +430,46 @@ template <int CH, int BITS, int ROT = 0>
 BCN_HD void opt_endpoints(const uint32_t px[16], const uint32_t qblock[2], uint32_t mask,
                           float e0[4], float e1[4], uint32_t sh = 0) {
   constexpr int L1 = (1 << BITS) - 1;
-  int sum_q = 0, sum_qq = 0, cnt = 0, sum[4] = {0, 0, 0, 0}, atb1[4] = {0, 0, 0, 0};
+  // Four texels at a time, as in moments(): channel-major bytes of the masked
+  // texels, index bytes, and one v_dot4_u32_u8 per sum.  x = L1 - q is
+  // q ^ L1 (q <= L1 = 2^BITS - 1).
+  const int cnt = __builtin_popcount(mask & 0xFFFFu);
+  uint32_t uq = 0, uqq = 0, us[4] = {0, 0, 0, 0}, ua[4] = {0, 0, 0, 0};
 #pragma unroll
-  for (int k = 0; k < 16; k++) {
-    int f = (int)((mask >> k) & 1u);
-    int q = (int)((qblock[k >> 3] >> (4 * (k & 7))) & 15u);
-    int qf = f ? q : 0;
-    int xf = f ? L1 - q : 0;
-    sum_q += qf;
-    sum_qq += qf * qf;
-    cnt += f;
-    uint32_t t = texel<ROT>(px, k, sh);
+  for (int g = 0; g < 4; g++) {
+    uint32_t t[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int k = 4 * g + i;
+      t[i] = ((mask >> k) & 1u) ? texel<ROT>(px, k, sh) : 0u;
+    }
+    const uint32_t a = perm_b32(t[1], t[0], 0x06020400u), b = perm_b32(t[1], t[0], 0x07030501u);
+    const uint32_t c = perm_b32(t[3], t[2], 0x06020400u), d = perm_b32(t[3], t[2], 0x07030501u);
+    uint32_t ch[4];
+    ch[0] = perm_b32(c, a, 0x05040100u);
+    ch[1] = perm_b32(d, b, 0x05040100u);
+    ch[2] = perm_b32(c, a, 0x07060302u);
+    ch[3] = CH == 4 ? perm_b32(d, b, 0x07060302u) : 0u;
+    // Index nibbles of the group's texels to bytes, and their mask.
+    const uint32_t nib = (qblock[g >> 1] >> (16 * (g & 1))) & 0xFFFFu;
+    const uint32_t qb = (nib & 0xFu) | ((nib & 0xF0u) << 4) | ((nib & 0xF00u) << 8) | ((nib & 0xF000u) << 12);
+    const uint32_t mb = ((((mask >> (4 * g)) & 0xFu) * 0x00204081u) & 0x01010101u) * 0xFFu;
+    const uint32_t qm = qb & mb, xb = qb ^ (0x01010101u * (uint32_t)L1);
+    uq = udot4(qm, 0x01010101u, uq);
+    uqq = udot4(qm, qm, uqq);
 #pragma unroll
     for (int p = 0; p < CH; p++) {
-      int v = f ? chi(t, p) : 0;
-      sum[p] += v;
-      atb1[p] += mul24(xf, v);
+      us[p] = udot4(ch[p], 0x01010101u, us[p]);
+      ua[p] = udot4(ch[p], xb, ua[p]);
     }
+  }
+  // Bounds: sum_q <= 240, sum_qq <= 3600, sum <= 4080, atb1 <= 61,200.
+  const int sum_q = (int)uq, sum_qq = (int)uqq;
+  int sum[4], atb1[4];
+#pragma unroll
+  for (int p = 0; p < 4; p++) {
+    sum[p] = (int)us[p];
+    atb1[p] = (int)ua[p];
   }
   int cxx = mul24i(cnt, L1 * L1) - mul24i(2 * L1, sum_q) + sum_qq;
   int cyy = sum_qq;
@@ -495,7 +518,7 @@ BCN_HD int channel_quant(const uint32_t px[16], uint32_t sh, int e0, int e1, uin
     const int cvk = chan(px, k, sh);
     float proj = div_rcp((float)(cvk - e0), den, rden);
     int q1 = qclamp(proj * (float)L + 0.5f, 1.0f, (float)(L - 1));
-    int w0 = weight<BITS>(q1 - 1), w1 = weight<BITS>(q1);
+    int w0 = weight_mad<BITS>(q1 - 1), w1 = weight_mad<BITS>(q1);
     // Interpolation as in block_quant: a + ((w * (b - a) + 32) >> 6).
     int t0 = e0 - cvk + ((mul24(w0, e1 - e0) + 32) >> 6);
     int t1 = e0 - cvk + ((mul24(w1, e1 - e0) + 32) >> 6);

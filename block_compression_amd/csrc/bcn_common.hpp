@@ -285,6 +285,29 @@ BCN_HD int weight(int i) {
   return BITS == 2 ? weight2(i) : (BITS == 3 ? weight3(i) : weight4(i));
 }
 
+// The same weights as one multiply-add and shift: round(64 i / m) with
+// m = 2^BITS - 1 equals (i * (4095 / m) + 32) >> 6 for every index of the
+// 2-, 3- and 4-bit tables (checked at compile time below).
+template <int BITS>
+BCN_HD int weight_mad(int i) {
+  constexpr int K = BITS == 2 ? 1365 : (BITS == 3 ? 585 : 273);
+  return (mul24(i, K) + 32) >> 6;
+}
+
+constexpr bool weight_mad_holds() {
+  constexpr int t2[4] = {0, 21, 43, 64};
+  constexpr int t3[8] = {0, 9, 18, 27, 37, 46, 55, 64};
+  constexpr int t4[16] = {0, 4, 9, 13, 17, 21, 26, 30, 34, 38, 43, 47, 51, 55, 60, 64};
+  for (int i = 0; i < 4; i++)
+    if (((i * 1365 + 32) >> 6) != t2[i]) return false;
+  for (int i = 0; i < 8; i++)
+    if (((i * 585 + 32) >> 6) != t3[i]) return false;
+  for (int i = 0; i < 16; i++)
+    if (((i * 273 + 32) >> 6) != t4[i]) return false;
+  return true;
+}
+static_assert(weight_mad_holds(), "BPTC weight closed form");
+
 // ---- partition tables -------------------------------------------------------
 //
 // The BPTC partition shapes (2 subsets: 64 shapes, 3 subsets: 64 shapes) as

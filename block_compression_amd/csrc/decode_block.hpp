@@ -120,24 +120,12 @@ BCN_HD int bptc_weight(int i) {
 }
 // The BPTC index weights are round(64 i / m) with m = 2^bits - 1, and for
 // m = 3, 7, 15 they equal (i * (4095 / m) + 32) >> 6 at every index (checked
-// at compile time below against the format's tables): one multiply instead
-// of a per-lane choice among three tables when `bits` differs by lane.
+// at compile time against the format's tables, weight_mad_holds in
+// bcn_common.hpp): one multiply instead of a per-lane choice among three
+// tables when `bits` differs by lane.
 BCN_HD int bptc_weight_mul(int bits) { return bits == 2 ? 1365 : (bits == 3 ? 585 : 273); }
 BCN_HD int bptc_weight_rt(int bits, int i) { return (mul24(i, bptc_weight_mul(bits)) + 32) >> 6; }
 
-constexpr bool bptc_weight_form_holds() {
-  constexpr int t2[4] = {0, 21, 43, 64};
-  constexpr int t3[8] = {0, 9, 18, 27, 37, 46, 55, 64};
-  constexpr int t4[16] = {0, 4, 9, 13, 17, 21, 26, 30, 34, 38, 43, 47, 51, 55, 60, 64};
-  for (int i = 0; i < 4; i++)
-    if (((i * 1365 + 32) >> 6) != t2[i]) return false;
-  for (int i = 0; i < 8; i++)
-    if (((i * 585 + 32) >> 6) != t3[i]) return false;
-  for (int i = 0; i < 16; i++)
-    if (((i * 273 + 32) >> 6) != t4[i]) return false;
-  return true;
-}
-static_assert(bptc_weight_form_holds(), "BPTC weight closed form");
 
 // Two 16-bit lanes of (a * (64 - w) + b * w + 32) >> 6 (block.rs:1256-1258):
 // lo/hi/w hold one channel per half-word.  Every term stays below 2^16
