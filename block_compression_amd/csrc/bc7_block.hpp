@@ -90,7 +90,7 @@ BCN_HD void compute_axis(float axis[4], const float c[10], int iters) {
       // The reference starts the sum at 0.0; 0.0 + x == x for x = sq(..) >= 0.
       float norm_sq = sq(x0) + sq(x1) + sq(x2);
       if (CH == 4) norm_sq = norm_sq + sq(x3);
-      float rnorm = 1.0f / sqrtf(norm_sq);
+      float rnorm = rsqrt_cr(norm_sq);
       a0 *= rnorm;
       a1 *= rnorm;
       a2 *= rnorm;
@@ -763,7 +763,7 @@ BCN_HD void partition_bounds(const uint32_t px[16], uint16_t* keys, int stride) 
       }
       float sq_sum = sq(a0) + sq(a1) + sq(a2);
       if (CH == 4) sq_sum = sq_sum + sq(a3);
-      float lambda = sqrtf(sq_sum);
+      float lambda = sqrt_cr(sq_sum);
       float bb = c[0] + c[4] + c[7];
       if (CH == 4) bb = bb + c[9];
       bb = bb - lambda;
@@ -771,7 +771,7 @@ BCN_HD void partition_bounds(const uint32_t px[16], uint16_t* keys, int stride) 
       // `bound += b`: the first add is 0.0 + b == b for b >= +0.0.
       bound = side == 0 ? bb : bound + bb;
     }
-    float bound12 = sqrtf(bound) * 256.0f;
+    float bound12 = sqrt_cr(bound) * 256.0f;
     keys[part * stride] = (uint16_t)(int)bound12;  // finite, 0 <= bound12 < 1450
   }
 }

@@ -217,6 +217,28 @@ BCN_HD float rcp_cr(float x) {
 #endif
 }
 
+// 1.0f / sqrtf(x), both correctly rounded, for every x.  For x in
+// [2^-100, 2^128) the square root is the hardware one with the +-1 ulp FMA
+// correction of sqrt_cr (exact from 2^-104 up, exhaustive GPU sweep) and lies
+// in [2^-50, 2^64], where the hardware reciprocal with one Newton step is the
+// correctly rounded 1/s (same sweep); other inputs take the compiler's
+// sequences, behind one branch.
+BCN_HD float rsqrt_cr(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if (!(x >= 0x1p-100f && x <= 0x1.fffffep127f)) return 1.0f / sqrtf(x);
+  float s = __builtin_amdgcn_sqrtf(x);
+  const uint32_t b = __float_as_uint(s);
+  const float sd = __uint_as_float(b - 1u), su = __uint_as_float(b + 1u);
+  const float rd = __builtin_fmaf(-sd, s, x), ru = __builtin_fmaf(-su, s, x);
+  s = rd <= 0.0f ? sd : s;
+  s = ru > 0.0f ? su : s;
+  const float y = __builtin_amdgcn_rcpf(s);
+  return __builtin_fmaf(__builtin_fmaf(-s, y, 1.0f), y, y);
+#else
+  return 1.0f / sqrtf(x);
+#endif
+}
+
 // x / 255.0f and x / 65535.0f for every finite x (see div_rcp).
 BCN_HD float div255(float x) { return div_rcp(x, 255.0f, 1.0f / 255.0f); }
 BCN_HD float div65535(float x) { return div_rcp(x, 65535.0f, 1.0f / 65535.0f); }

@@ -6,8 +6,8 @@
 // the library uses them; mode 1 their unguarded fast paths, to show where the
 // guards are needed.  The second check sweeps the BC6H -> RGBA8 decode's sRGB8
 // code (srgb8_estimate / srgb8_code in decode_block.hpp) over every
-// non-negative finite half.  Test infrastructure only: the library does not
-// link this.
+// non-negative finite half; the third, rsqrt_cr against 1.0f / sqrtf(x).
+// Test infrastructure only: the library does not link this.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -89,6 +89,35 @@ extern "C" int srgb8_check(const uint16_t* table, uint32_t n, uint8_t* est, uint
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e == hipSuccess) e = hipMemcpy(est, d, n, hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(code, d + 0x7C00, n, hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  return (int)e;
+}
+
+// ---- rsqrt_cr (bcn_common.hpp) ------------------------------------------------
+// For every float bit pattern in [lo, hi): rsqrt_cr(x) against the compiler's
+// correctly rounded 1.0f / sqrtf(x), mismatches counted per exponent field.
+
+__global__ void rsqrt_kernel(uint32_t lo, uint32_t hi, unsigned long long* bad) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t u = (uint64_t)lo + blockIdx.x * blockDim.x + threadIdx.x; u < hi; u += stride) {
+    const uint32_t bits = (uint32_t)u;
+    const float x = __uint_as_float(bits);
+    if (__float_as_uint(bcn::rsqrt_cr(x)) != __float_as_uint(1.0f / sqrtf(x)))
+      atomicAdd(&bad[(bits >> 23) & 0xFFu], 1ull);
+  }
+}
+
+extern "C" int rsqrt_check(uint32_t lo, uint32_t hi, unsigned long long* bad) {
+  unsigned long long* d = nullptr;
+  hipError_t e = hipMalloc(&d, 256 * sizeof(unsigned long long));
+  if (e != hipSuccess) return (int)e;
+  e = hipMemset(d, 0, 256 * sizeof(unsigned long long));
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(rsqrt_kernel, dim3(8192), dim3(256), 0, 0, lo, hi, d);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(bad, d, 256 * sizeof(unsigned long long), hipMemcpyDeviceToHost);
   (void)hipFree(d);
   return (int)e;
 }

@@ -1,4 +1,5 @@
-"""The kernels' square root and reciprocal (sqrt_cr, rcp_cr in
+"""The kernels' square root, reciprocal and reciprocal square root (sqrt_cr,
+rcp_cr, rsqrt_cr in
 block_compression_amd/csrc/bcn_common.hpp) equal the compiler's correctly
 rounded sqrtf and 1.0f / x on every finite float, swept exhaustively on the
 GPU, and the unguarded reciprocal (rcp_normal) equals 1.0f / x on every normal
@@ -31,6 +32,16 @@ def test_guarded_sqrt_and_reciprocal_exact_on_every_finite_float():
         print(f"fastmath: sweep {lo:#x}..{hi:#x}", flush=True)
         s, r = _sweep(lo, hi, 0)
         assert s.sum() == 0 and r.sum() == 0
+
+
+def test_rsqrt_exact_on_every_non_negative_float():
+    """rsqrt_cr (the power iteration's renormalisation) equals 1.0f / sqrtf(x)
+    bit for bit on every non-negative float, +inf and NaN patterns included."""
+    print("fastmath: rsqrt sweep", flush=True)
+    lib = ctypes.CDLL(LIB)
+    bad = (ctypes.c_ulonglong * 256)()
+    assert lib.rsqrt_check(ctypes.c_uint32(0), ctypes.c_uint32(0x80000000), bad) == 0
+    assert sum(bad[:]) == 0
 
 
 def test_unguarded_fast_paths_differ_only_where_guarded():
