@@ -269,7 +269,9 @@ BCN_HD void quant_pair_0367(const float e0[4], const float e1[4], int q0[4], int
     for (int b = 0; b < 2; b++)
 #pragma unroll
       for (int p = 0; p < QCH; p++) {
-        float x = (div255(ep[p]) * (float)L2 - (float)b) / 2.0f + 0.5f;
+        // y / 2 + 0.5 == fma(y, 0.5, 0.5): halving is exact, or |y| < 2^-125
+        // and both round to 0.5.
+        float x = __builtin_fmaf(div255(ep[p]) * (float)L2 - (float)b, 0.5f, 0.5f);
         qb[b][p] = qclamp(x, 0.0f, (float)(L - 1)) * 2 + b;
         eb[b][p] = MODE == 0 ? (float)unpack_to_byte(qb[b][p], 5) : (float)qb[b][p];
       }
@@ -304,7 +306,7 @@ BCN_HD void quant_pair_1(const float e0[4], const float e1[4], int q0[4], int q1
 #pragma unroll
       for (int p = 0; p < 3; p++) {
         float v = (i == 0 ? e0[p] : e1[p]);
-        float x = (div255(v) * 127.0f - (float)b) / 2.0f + 0.5f;
+        float x = __builtin_fmaf(div255(v) * 127.0f - (float)b, 0.5f, 0.5f);  // as in quant_pair_0367
         qb[b][i][p] = qclamp(x, 0.0f, 63.0f) * 2 + b;
         eb[b][i][p] = (float)unpack_to_byte(qb[b][i][p], 7);
       }
@@ -384,6 +386,7 @@ BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pat
     frcp[j] = rcp_normal(fdiv[j]);
   }
   const uint32_t ga_mask = CH == 4 ? 0x00FF00FFu : 0x000000FFu;
+  const uint32_t m64 = fresh(0xFFC0FFC0u);
   int total = 0;
   qblock[0] = 0;
   qblock[1] = 0;
@@ -391,33 +394,36 @@ BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pat
   for (int k = 0; k < 16; k++) {
     int j = NSUB == 1 ? 0 : (int)((pattern >> (2 * k)) & 3u);
     // Subset selects with constant indices keep the arrays in registers.
-#define BCN_PICK(v) (NSUB == 1 ? v[0] : (j == 0 ? v[0] : (j == 1 ? v[1] : v[2])))
+#define BCN_PICK(v) (NSUB == 1 ? v[0] : (NSUB == 2 ? (j == 0 ? v[0] : v[1]) : (j == 0 ? v[0] : (j == 1 ? v[1] : v[2]))))
     const uint32_t erb = BCN_PICK(ea_rb), ega = BCN_PICK(ea_ga);
     const uint32_t drb = BCN_PICK(de_rb), dga = BCN_PICK(de_ga);
+    const float div = BCN_PICK(fdiv), rdiv = BCN_PICK(frcp);
 #undef BCN_PICK
-    float div = NSUB == 1 ? fdiv[0] : (j == 0 ? fdiv[0] : (j == 1 ? fdiv[1] : fdiv[2]));
-    float rdiv = NSUB == 1 ? frcp[0] : (j == 0 ? frcp[0] : (j == 1 ? frcp[1] : frcp[2]));
     uint32_t t = texel<ROT>(px, k, sh);
-    // base = endpoint 0 - texel, per channel.
-    const uint32_t brb = pk_sub16(erb, t & 0x00FF00FFu), bga = pk_sub16(ega, (t >> 8) & ga_mask);
-    // proj = sum (texel - e0) * (e1 - e0) = -sum base * diff.
-    int proj = -dot2_16(bga, dga, dot2_16(brb, drb, 0));
+    // b = texel - endpoint 0, per channel.
+    const uint32_t brb = pk_sub16(t & 0x00FF00FFu, erb), bga = pk_sub16((t >> 8) & ga_mask, ega);
+    // proj = sum (texel - e0) * (e1 - e0).
+    int proj = dot2_16(bga, dga, dot2_16(brb, drb, 0));
     // |proj| <= |texel - e0| |e1 - e0| <= 510 sqrt(div) (Cauchy-Schwarz).
     float fp = div_rcp((float)proj, div, rdiv);
-    int q1 = qclamp(fp * (float)L + 0.5f, 1.0f, (float)(L - 1));
-    const uint32_t w0 = (uint32_t)weight_mad<BITS>(q1 - 1) * 0x10001u;
-    const uint32_t w1 = (uint32_t)weight_mad<BITS>(q1) * 0x10001u;
+    // fp * L is exact (a power-of-two multiple of a quotient in [-510, 510]),
+    // so the fused form rounds once, as fp * L + 0.5 does.
+    int q1 = qclamp(__builtin_fmaf(fp, (float)L, 0.5f), 1.0f, (float)(L - 1));
+    const uint32_t wp = weight_pair<BITS>(q1);
     // The reference's ((64 - w) * a + w * b + 32) >> 6 with b = a + diff: the
     // 64 * a term is a multiple of 64 and >> is floor division, so it equals
-    // a + ((w * diff + 32) >> 6); minus the texel, base + that.
-    const uint32_t t0rb = pk_add16(brb, pk_wdelta16(w0, drb)), t0ga = pk_add16(bga, pk_wdelta16(w0, dga));
-    const uint32_t t1rb = pk_add16(brb, pk_wdelta16(w1, drb)), t1ga = pk_add16(bga, pk_wdelta16(w1, dga));
-    int err0 = dot2_16(t0ga, t0ga, dot2_16(t0rb, t0rb, 0));
-    int err1 = dot2_16(t1ga, t1ga, dot2_16(t1rb, t1rb, 0));
-    bool lo = err0 < err1;
-    int q = lo ? q1 - 1 : q1;
-    total += lo ? err0 : err1;
-    qblock[k >> 3] |= (uint32_t)q << (4 * (k & 7));
+    // a + ((w * diff + 32) >> 6); minus the texel (= a + b) that is
+    // (w * diff + 32 - 64 * b) >> 6, with |w * diff|, |64 * b| <= 16320.
+    const uint32_t c_rb = pk_round_off16(brb, m64), c_ga = pk_round_off16(bga, m64);
+    const uint32_t t0rb = pk_wmad16<0>(wp, drb, c_rb), t0ga = pk_wmad16<0>(wp, dga, c_ga);
+    const uint32_t t1rb = pk_wmad16<1>(wp, drb, c_rb), t1ga = pk_wmad16<1>(wp, dga, c_ga);
+    // The error sums continue the running total (every partial sum < 2^23),
+    // so the smaller of the two is the new total.
+    const int err0 = dot2_16(t0ga, t0ga, dot2_16(t0rb, t0rb, total));
+    const int err1 = dot2_16(t1ga, t1ga, dot2_16(t1rb, t1rb, total));
+    const bool lo = err0 < err1;
+    total = lo ? err0 : err1;
+    qblock[k >> 3] |= (uint32_t)(q1 - (int)lo) << (4 * (k & 7));
   }
   return total;
 }
@@ -517,14 +523,20 @@ BCN_HD int channel_quant(const uint32_t px[16], uint32_t sh, int e0, int e1, uin
   for (int k = 0; k < 16; k++) {
     const int cvk = chan(px, k, sh);
     float proj = div_rcp((float)(cvk - e0), den, rden);
-    int q1 = qclamp(proj * (float)L + 0.5f, 1.0f, (float)(L - 1));
+    // proj * L is exact (|proj| <= 255000, a power-of-two factor), so the
+    // fused form rounds once, as proj * L + 0.5 does.
+    int q1 = qclamp(__builtin_fmaf(proj, (float)L, 0.5f), 1.0f, (float)(L - 1));
     int w0 = weight_mad<BITS>(q1 - 1), w1 = weight_mad<BITS>(q1);
-    // Interpolation as in block_quant: a + ((w * (b - a) + 32) >> 6).
-    int t0 = e0 - cvk + ((mul24(w0, e1 - e0) + 32) >> 6);
-    int t1 = e0 - cvk + ((mul24(w1, e1 - e0) + 32) >> 6);
-    int err0 = mul24(t0, t0), err1 = mul24(t1, t1);
-    bool lo = err0 < err1;
-    total += lo ? err0 : err1;
+    // Interpolated minus texel as in block_quant: a + ((w * (b - a) + 32) >> 6)
+    // - texel = (w * (b - a) + 32 - 64 * (texel - a)) >> 6.
+    const int c = mul24(cvk - e0, -64) + 32;
+    int t0 = (mul24(w0, e1 - e0) + c) >> 6;
+    int t1 = (mul24(w1, e1 - e0) + c) >> 6;
+    // Each error on top of the running total (sums <= 16 * 255^2); the
+    // smaller is the new total.
+    const int err0 = mul24(t0, t0) + total, err1 = mul24(t1, t1) + total;
+    const bool lo = err0 < err1;
+    total = lo ? err0 : err1;
     qblock[k >> 3] |= (uint32_t)(lo ? q1 - 1 : q1) << (4 * (k & 7));
   }
   return total;

@@ -316,6 +316,49 @@ BCN_HD int weight_mad(int i) {
   return (mul24(i, K) + 32) >> 6;
 }
 
+// Weights of indices q - 1 and q (1 <= q < 2^BITS) as the low and high 16-bit
+// lanes: (q - 1) | q << 16 is one 24-bit multiply-add, then the closed form
+// lane-wise (i * K + 32 <= 4127 fits the lanes).
+template <int BITS>
+BCN_HD uint32_t weight_pair(int q) {
+  constexpr int K = BITS == 2 ? 1365 : (BITS == 3 ? 585 : 273);
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+  const u16x2 qq = __builtin_bit_cast(u16x2, (uint32_t)(mul24(q, 0x10001) - 1));
+  const u16x2 k = {K, K}, c32 = {32, 32}, c6 = {6, 6};
+  return __builtin_bit_cast(uint32_t, (u16x2)((qq * k + c32) >> c6));
+#else
+  return pk16(weight_mad<BITS>(q - 1), weight_mad<BITS>(q));
+#endif
+}
+
+// Lane-wise (w * d + c) >> 6 (arithmetic shift) with w lane H of wp in both
+// lanes.  Callers keep |w * d + c| < 2^15; the 16-bit multiply-add wraps
+// modulo 2^16, which is exact for such results.
+template <int H>
+BCN_HD uint32_t pk_wmad16(uint32_t wp, uint32_t d, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const bcn_s16x2 w = H ? as_s16x2(wp).yy : as_s16x2(wp).xx, c6 = {6, 6};
+  return as_u32((w * as_s16x2(d) + as_s16x2(c)) >> c6);
+#else
+  const int w = lane16(wp, H);
+  return pk16((w * lane16(d, 0) + lane16(c, 0)) >> 6, (w * lane16(d, 1) + lane16(c, 1)) >> 6);
+#endif
+}
+
+// Lane-wise 32 - 64 * b for |b| <= 255: the rounding term of pk_wmad16 with
+// b whole units taken off.  m64 is 0xFFC0FFC0 (-64 in both lanes) held in a
+// register: with a visible constant the compiler splits the multiply-add into
+// a shift and a subtract.
+BCN_HD uint32_t pk_round_off16(uint32_t b, uint32_t m64) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const bcn_s16x2 c32 = {32, 32};
+  return as_u32(as_s16x2(b) * as_s16x2(m64) + c32);
+#else
+  return pk16(32 + lane16(m64, 0) * lane16(b, 0), 32 + lane16(m64, 1) * lane16(b, 1));
+#endif
+}
+
 constexpr bool weight_mad_holds() {
   constexpr int t2[4] = {0, 21, 43, 64};
   constexpr int t3[8] = {0, 9, 18, 27, 37, 46, 55, 64};
