@@ -700,22 +700,132 @@ BCN_HD void finish_mode01237(const uint32_t px[16], const Bc7Params& s, PartFit 
   if (better(best, err, order)) store_partfit<MODE>(best, bf, err, part_id, order);
 }
 
-// bc7_enc_mode02's calls (bc7.rs:612-620): every partition 0..count-1, in order.
+// ---- subset fits shared between the 3-subset shapes of modes 0 and 2 -------------
+//
+// A subset's PCA endpoints and their quantisation depend only on its texel
+// mask.  Of the 48 subset masks of the 16 shapes mode 0 tries, 12 repeat an
+// earlier one (52 of the 192 of mode 2's 64 shapes).  Every lane runs the same
+// shape in these loops, so a compile-time plan says per shape and subset
+// whether to fit and keep the result for a later repeat, only fit, or take a
+// kept result.  A kept result is its two endpoints' 5-bit channel codes, 15
+// bits each, in two entries of the lane's partition-key row, which modes
+// 1/3/7 use only afterwards.
+enum : uint8_t { kFitOnly = 0, kFitKeep = 1, kReuse = 2 };
+
+struct SubsetPlan {
+  uint8_t op[64][3];    // kFitOnly, kFitKeep or kReuse
+  uint8_t slot[64][3];  // kept result in row entries 2*slot and 2*slot + 1
+  int slots;            // slots held at once, at most
+};
+
+// Shapes 64 .. 64 + count - 1 in order.  A kept result stays until the last
+// repeat of its mask has read it; that slot may then be reused by a later
+// subset of the same shape, whose write follows the read.
+constexpr SubsetPlan make_subset_plan(int count) {
+  SubsetPlan pl{};
+  uint32_t m[64][3] = {};
+  for (int p = 0; p < count; p++)
+    for (int j = 0; j < 3; j++) m[p][j] = shape_mask(64 + p, j);
+  bool busy[32] = {};
+  int held = 0;
+  for (int p = 0; p < count; p++)
+    for (int j = 0; j < 3; j++) {
+      int prev = -1;
+      bool later = false;
+      for (int q = 0; q < count; q++)
+        for (int i = 0; i < 3; i++)
+          if (m[q][i] == m[p][j]) {
+            if (q < p && pl.op[q][i] != kFitOnly) prev = pl.slot[q][i];
+            if (q > p) later = true;
+          }
+      if (prev >= 0) {
+        pl.op[p][j] = kReuse;
+        pl.slot[p][j] = (uint8_t)prev;
+        if (!later) {
+          busy[prev] = false;
+          held--;
+        }
+      } else if (later) {
+        int f = 0;
+        while (busy[f]) f++;
+        busy[f] = true;
+        held++;
+        pl.slots = held > pl.slots ? held : pl.slots;
+        pl.op[p][j] = kFitKeep;
+        pl.slot[p][j] = (uint8_t)f;
+      }
+    }
+  return pl;
+}
+
+constexpr SubsetPlan kPlanMode0Host = make_subset_plan(16);
+constexpr SubsetPlan kPlanMode2Host = make_subset_plan(64);
+// 6 and 21 slots: 12 and 42 of the row's 64 entries.
+static_assert(kPlanMode0Host.slots * 2 <= 64 && kPlanMode2Host.slots * 2 <= 64,
+              "kept subset fits must fit the 64-entry partition-key row");
+BCN_TABLE SubsetPlan kPlanMode0 = make_subset_plan(16);
+BCN_TABLE SubsetPlan kPlanMode2 = make_subset_plan(64);
+
+// Three 5-bit channel codes in 15 bits, and back to one byte per channel.
+BCN_HD uint32_t code15(const int q[4]) {
+  return (uint32_t)q[0] | ((uint32_t)q[1] << 5) | ((uint32_t)q[2] << 10);
+}
+BCN_HD uint32_t code15_bytes(uint32_t c) {
+  return (c & 0x1Fu) | ((c << 3) & 0x1F00u) | ((c << 6) & 0x1F0000u);
+}
+
+// fit_partition for shape 64 + part of mode 0 or 2 following the plan; both
+// modes dequantise their 5-bit codes with unpack_to_byte(q, 5).
+template <int MODE>
+BCN_HD void fit_partition_planned(const uint32_t px[16], int part, uint16_t* row, int stride,
+                                  PartFit& f) {
+  constexpr int BITS = MODE == 0 ? 3 : 2;
+  const int part_id = part + 64;
+  int d[3][2][4];
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    const int op = MODE == 0 ? kPlanMode0.op[part][j] : kPlanMode2.op[part][j];
+    const int slot = MODE == 0 ? kPlanMode0.slot[part][j] : kPlanMode2.slot[part][j];
+    if (op == kReuse) {
+      const uint32_t c0 = row[(2 * slot) * stride], c1 = row[(2 * slot + 1) * stride];
+      f.q[j][0] = code15_bytes(c0);
+      f.q[j][1] = code15_bytes(c1);
+#pragma unroll
+      for (int p = 0; p < 3; p++) {
+        d[j][0][p] = unpack_to_byte((int)((c0 >> (5 * p)) & 0x1Fu), 5);
+        d[j][1][p] = unpack_to_byte((int)((c1 >> (5 * p)) & 0x1Fu), 5);
+      }
+    } else {
+      float e0[4], e1[4];
+      bc7_segment<3>(px, part_mask(part_id, j), e0, e1);
+      int q0[4], q1[4];
+      quant_pair<MODE, 3>(e0, e1, q0, q1, d[j][0], d[j][1]);
+      f.q[j][0] = pack4(q0, 3);
+      f.q[j][1] = pack4(q1, 3);
+      if (op == kFitKeep) {
+        row[(2 * slot) * stride] = (uint16_t)code15(q0);
+        row[(2 * slot + 1) * stride] = (uint16_t)code15(q1);
+      }
+    }
+  }
+  f.err = block_quant<3, BITS, 3>(px, d, kPart.subset[part_id], f.qblock);
+}
+
+// bc7_enc_mode02's calls (bc7.rs:612-620): every partition 0..count-1, in
+// order (count is 16 for mode 0 and 64 for mode 2).
 template <int MODE>
 BCN_HD void enc_mode02(const uint32_t px[16], const Bc7Params& s, int count, int opaque_err,
-                       int order, Bc7Best& best) {
-  constexpr int PAIRS = (MODE == 0 || MODE == 2) ? 3 : 2;
+                       int order, uint16_t* row, int stride, Bc7Best& best) {
   PartFit bf;
   bf.err = 0x7FFFFFFF;
   int best_part = 0;
 #pragma unroll 1
   for (int part = 0; part < count; part++) {
-    int part_id = PAIRS == 3 ? part + 64 : part;
     PartFit f;
-    fit_partition<MODE>(px, part_id, f);
+    fit_partition_planned<MODE>(px, part, row, stride, f);
     if (f.err < bf.err) {
       bf = f;
-      best_part = part_id;
+      best_part = part + 64;
     }
   }
   finish_mode01237<MODE>(px, s, bf, best_part, opaque_err, order, best);
@@ -1146,8 +1256,8 @@ BCN_HD void encode_bc7_block(const uint32_t px[16], const Bc7Params& s, uint16_t
 #pragma unroll
   for (int k = 0; k < 16; k++) t[k] = fresh(px[k]);
   if (s.mode_selection[0] != 0) {
-    enc_mode02<0>(t, s, 16, opaque_err, 0, best);
-    if (s.skip_mode2 == 0) enc_mode02<2>(t, s, 64, opaque_err, 1, best);
+    enc_mode02<0>(t, s, 16, opaque_err, 0, keys, stride, best);
+    if (s.skip_mode2 == 0) enc_mode02<2>(t, s, 64, opaque_err, 1, keys, stride, best);
   }
 #pragma unroll
   for (int k = 0; k < 16; k++) t[k] = fresh(t[k]);

@@ -321,8 +321,8 @@ BCN_HD int weight_mad(int i) {
 // lane-wise (i * K + 32 <= 4127 fits the lanes).
 template <int BITS>
 BCN_HD uint32_t weight_pair(int q) {
-  constexpr int K = BITS == 2 ? 1365 : (BITS == 3 ? 585 : 273);
 #if defined(__HIP_DEVICE_COMPILE__)
+  constexpr int K = BITS == 2 ? 1365 : (BITS == 3 ? 585 : 273);
   typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
   const u16x2 qq = __builtin_bit_cast(u16x2, (uint32_t)(mul24(q, 0x10001) - 1));
   const u16x2 k = {K, K}, c32 = {32, 32}, c6 = {6, 6};
@@ -386,8 +386,7 @@ struct PartitionTables {
   uint8_t anchor2[128];  // anchor texel of subset 2 (3-subset shapes; 0 otherwise)
 };
 
-constexpr PartitionTables make_partition_tables() {
-  constexpr uint32_t shapes[128] = {
+constexpr uint32_t kShapes[128] = {
       0x50505050u, 0x40404040u, 0x54545454u, 0x54505040u, 0x50404000u, 0x55545450u, 0x55545040u,
       0x54504000u, 0x50400000u, 0x55555450u, 0x55544000u, 0x54400000u, 0x55555440u, 0x55550000u,
       0x55555500u, 0x55000000u, 0x55150100u, 0x00004054u, 0x15010000u, 0x00405054u, 0x00004050u,
@@ -407,6 +406,16 @@ constexpr PartitionTables make_partition_tables() {
       0xAA444444u, 0x54A854A8u, 0x95809580u, 0x96969600u, 0xA85454A8u, 0x80959580u, 0xAA141414u,
       0x96960000u, 0xAAAA1414u, 0xA05050A0u, 0xA0A5A5A0u, 0x96000000u, 0x40804080u, 0xA9A8A9A8u,
       0xAAAAAA44u, 0x2A4A5254u};
+
+// Texel mask (16 bits) of subset j of shape i.
+constexpr uint32_t shape_mask(int i, int j) {
+  uint32_t m = 0;
+  for (int k = 0; k < 16; k++)
+    if (((kShapes[i] >> (2 * k)) & 3u) == (uint32_t)j) m |= 1u << k;
+  return m;
+}
+
+constexpr PartitionTables make_partition_tables() {
   constexpr uint8_t a1[128] = {
       15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 2,  8,  2,  2,  8,
       8,  15, 2,  8,  2,  2,  8,  8,  2,  2,  15, 15, 6,  8,  2,  8,  15, 15, 2,  8,  2,  2,
@@ -420,14 +429,8 @@ constexpr PartitionTables make_partition_tables() {
                               15, 3,  15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 3,  15, 15, 8};
   PartitionTables t{};
   for (int i = 0; i < 128; i++) {
-    t.subset[i] = shapes[i];
-    uint32_t m0 = 0, m1 = 0;
-    for (int k = 0; k < 16; k++) {
-      uint32_t s = (shapes[i] >> (2 * k)) & 3u;
-      if (s == 0) m0 |= 1u << k;
-      if (s == 1) m1 |= 1u << k;
-    }
-    t.mask01[i] = m0 | (m1 << 16);
+    t.subset[i] = kShapes[i];
+    t.mask01[i] = shape_mask(i, 0) | (shape_mask(i, 1) << 16);
     t.anchor1[i] = a1[i];
     t.anchor2[i] = i < 64 ? 0 : a2[i - 64];
   }
