@@ -516,28 +516,34 @@ BCN_HD int channel_quant(const uint32_t px[16], uint32_t sh, int e0, int e1, uin
   constexpr int L = 1 << BITS;
   // |den| in [0.001, 255.001]
   float den = (float)(e1 - e0) + 0.001f, rden = rcp_normal(den);
+  const float fe0 = (float)e0;
   qblock[0] = 0;
   qblock[1] = 0;
+  // Both candidates' interpolated-minus-texel values in the two 16-bit lanes:
+  // as in block_quant, a + ((w * (b - a) + 32) >> 6) - texel =
+  // (w * (b - a) + (32 + 64 * a) - 64 * texel) >> 6, all terms within 2^15.
+  const uint32_t de2 = pk16(e1 - e0, e1 - e0), base2 = pk16(32 + 64 * e0, 32 + 64 * e0);
+  const uint32_t m64 = fresh(0xFFC0FFC0u);  // -64 per lane, see pk_round_off16
+  // Byte sh / 8 of a texel in both lanes.
+  const uint32_t sel = ((sh >> 3) * 0x00010001u) | 0x0C000C00u;
   int total = 0;
 #pragma unroll
   for (int k = 0; k < 16; k++) {
-    const int cvk = chan(px, k, sh);
-    float proj = div_rcp((float)(cvk - e0), den, rden);
+    const uint32_t cv2 = perm_b32(0u, fresh(px[k]), sel);
+    // (float)(texel - e0) as a difference of exact small integers.
+    float proj = div_rcp(chf(cv2, 0) - fe0, den, rden);
     // proj * L is exact (|proj| <= 255000, a power-of-two factor), so the
     // fused form rounds once, as proj * L + 0.5 does.
     int q1 = qclamp(__builtin_fmaf(proj, (float)L, 0.5f), 1.0f, (float)(L - 1));
-    int w0 = weight_mad<BITS>(q1 - 1), w1 = weight_mad<BITS>(q1);
-    // Interpolated minus texel as in block_quant: a + ((w * (b - a) + 32) >> 6)
-    // - texel = (w * (b - a) + 32 - 64 * (texel - a)) >> 6.
-    const int c = mul24(cvk - e0, -64) + 32;
-    int t0 = (mul24(w0, e1 - e0) + c) >> 6;
-    int t1 = (mul24(w1, e1 - e0) + c) >> 6;
-    // Each error on top of the running total (sums <= 16 * 255^2); the
-    // smaller is the new total.
-    const int err0 = mul24(t0, t0) + total, err1 = mul24(t1, t1) + total;
+    const uint32_t t = pk_mad_shr6(weight_pair<BITS>(q1), de2, pk_mad16(cv2, m64, base2));
+    // |t| <= 255, so the squares fit unsigned 16-bit lanes.  Each error on
+    // top of the running total (sums <= 16 * 255^2); the smaller is the new
+    // total.
+    const uint32_t sq = pk_sq16(t);
+    const int err0 = (int)(sq & 0xFFFFu) + total, err1 = (int)(sq >> 16) + total;
     const bool lo = err0 < err1;
     total = lo ? err0 : err1;
-    qblock[k >> 3] |= (uint32_t)(lo ? q1 - 1 : q1) << (4 * (k & 7));
+    qblock[k >> 3] |= (uint32_t)(q1 - (int)lo) << (4 * (k & 7));
   }
   return total;
 }
@@ -565,19 +571,34 @@ BCN_HD ChannelFit opt_channel(const uint32_t px[16], uint32_t sh, uint32_t refin
   int e0 = channel_qd<EPBITS>((float)mn, r.q0);
   int e1 = channel_qd<EPBITS>((float)mx, r.q1);
   r.err = channel_quant<BITS>(px, sh, e0, e1, r.qblock);
+  if (refine == 0) return r;
+  // The channel bytes of texels 4g .. 4g + 3 in one word per group, and their
+  // sum; the refinement's other sums are then one v_dot4_u32_u8 per group,
+  // as in opt_endpoints.
+  const uint32_t b = sh >> 3, sel = b | ((b + 4) << 8) | 0x0C0C0000u;
+  uint32_t cw[4], usum = 0;
+#pragma unroll
+  for (int g = 0; g < 4; g++) {
+    const uint32_t lo = perm_b32(fresh(px[4 * g + 1]), fresh(px[4 * g]), sel);
+    const uint32_t hi = perm_b32(fresh(px[4 * g + 3]), fresh(px[4 * g + 2]), sel);
+    cw[g] = perm_b32(hi, lo, 0x05040100u);
+    usum = udot4(cw[g], 0x01010101u, usum);
+  }
+  const int sum = (int)usum;  // <= 16 * 255
 #pragma unroll 1
   for (uint32_t it = 0; it < refine; it++) {
     // Sums <= 16*15*255; cxx, cyy, cxy, det exact as in opt_endpoints.
-    int sum_q = 0, sum_qq = 0, sum = 0, atb1 = 0;
+    uint32_t uq = 0, uqq = 0, ua = 0;
 #pragma unroll
-    for (int k = 0; k < 16; k++) {
-      int q = (int)((r.qblock[k >> 3] >> (4 * (k & 7))) & 15u);
-      sum_q += q;
-      sum_qq += q * q;
-      int v = chan(px, k, sh);
-      sum += v;
-      atb1 += mul24(L1 - q, v);
+    for (int g = 0; g < 4; g++) {
+      const uint32_t nib = (r.qblock[g >> 1] >> (16 * (g & 1))) & 0xFFFFu;
+      const uint32_t qb = (nib & 0xFu) | ((nib & 0xF0u) << 4) | ((nib & 0xF00u) << 8) | ((nib & 0xF000u) << 12);
+      uq = udot4(qb, 0x01010101u, uq);
+      uqq = udot4(qb, qb, uqq);
+      // L1 - q == q ^ L1 for q <= L1 = 2^BITS - 1.
+      ua = udot4(cw[g], qb ^ (0x01010101u * (uint32_t)L1), ua);
     }
+    const int sum_q = (int)uq, sum_qq = (int)uqq, atb1 = (int)ua;
     int cxx = 16 * L1 * L1 - mul24i(2 * L1, sum_q) + sum_qq;
     int cyy = sum_qq;
     int cxy = mul24i(L1, sum_q) - sum_qq;

@@ -359,6 +359,37 @@ BCN_HD uint32_t pk_round_off16(uint32_t b, uint32_t m64) {
 #endif
 }
 
+// Lane-wise (a * b + c) >> 6 (arithmetic shift) for |a * b + c| < 2^15.
+BCN_HD uint32_t pk_mad_shr6(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const bcn_s16x2 c6 = {6, 6};
+  return as_u32((as_s16x2(a) * as_s16x2(b) + as_s16x2(c)) >> c6);
+#else
+  return pk16((lane16(a, 0) * lane16(b, 0) + lane16(c, 0)) >> 6,
+              (lane16(a, 1) * lane16(b, 1) + lane16(c, 1)) >> 6);
+#endif
+}
+
+// Lane-wise a * b + c modulo 2^16 (v_pk_mad_u16).
+BCN_HD uint32_t pk_mad16(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return as_u32(as_s16x2(a) * as_s16x2(b) + as_s16x2(c));
+#else
+  return pk16(lane16(a, 0) * lane16(b, 0) + lane16(c, 0), lane16(a, 1) * lane16(b, 1) + lane16(c, 1));
+#endif
+}
+
+// Lane-wise a * a for |a| <= 255 (the square fits an unsigned 16-bit lane).
+BCN_HD uint32_t pk_sq16(uint32_t a) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+  const u16x2 v = __builtin_bit_cast(u16x2, a);
+  return __builtin_bit_cast(uint32_t, (u16x2)(v * v));
+#else
+  return pk16(lane16(a, 0) * lane16(a, 0), lane16(a, 1) * lane16(a, 1));
+#endif
+}
+
 constexpr bool weight_mad_holds() {
   constexpr int t2[4] = {0, 21, 43, 64};
   constexpr int t3[8] = {0, 9, 18, 27, 37, 46, 55, 64};
