@@ -156,6 +156,59 @@ BCN_HD Moments moments(const uint32_t px[16], uint32_t mask, uint32_t sh = 0) {
   return m;
 }
 
+// The texels as channel-major bytes: cm[g][p] holds channel p of texels
+// 4g .. 4g + 3 (the transpose moments() does per call).
+BCN_HD void channel_major(const uint32_t px[16], uint32_t cm[4][4]) {
+#pragma unroll
+  for (int g = 0; g < 4; g++) {
+    const uint32_t t0 = fresh(px[4 * g]), t1 = fresh(px[4 * g + 1]);
+    const uint32_t t2 = fresh(px[4 * g + 2]), t3 = fresh(px[4 * g + 3]);
+    const uint32_t a = perm_b32(t1, t0, 0x06020400u), b = perm_b32(t1, t0, 0x07030501u);
+    const uint32_t c = perm_b32(t3, t2, 0x06020400u), d = perm_b32(t3, t2, 0x07030501u);
+    cm[g][0] = perm_b32(c, a, 0x05040100u);
+    cm[g][1] = perm_b32(d, b, 0x05040100u);
+    cm[g][2] = perm_b32(c, a, 0x07060302u);
+    cm[g][3] = perm_b32(d, b, 0x07060302u);
+  }
+}
+
+// moments() from channel-major bytes: the masked channel bytes are one AND
+// with the group's byte mask, then one v_dot4_u32_u8 per sum.
+template <int CH>
+BCN_HD Moments moments_cm(const uint32_t cm[4][4], uint32_t mask) {
+  Moments m{};
+  m.n = __builtin_popcount(mask & 0xFFFFu);
+  uint32_t s[4] = {0, 0, 0, 0}, q[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int g = 0; g < 4; g++) {
+    const uint32_t mb = ((((mask >> (4 * g)) & 0xFu) * 0x00204081u) & 0x01010101u) * 0xFFu;
+    uint32_t x[4];
+#pragma unroll
+    for (int p = 0; p < CH; p++) x[p] = cm[g][p] & mb;
+    s[0] = udot4(x[0], 0x01010101u, s[0]);
+    s[1] = udot4(x[1], 0x01010101u, s[1]);
+    s[2] = udot4(x[2], 0x01010101u, s[2]);
+    q[0] = udot4(x[0], cm[g][0], q[0]);
+    q[1] = udot4(x[0], cm[g][1], q[1]);
+    q[2] = udot4(x[0], cm[g][2], q[2]);
+    q[4] = udot4(x[1], cm[g][1], q[4]);
+    q[5] = udot4(x[1], cm[g][2], q[5]);
+    q[7] = udot4(x[2], cm[g][2], q[7]);
+    if (CH == 4) {
+      s[3] = udot4(x[3], 0x01010101u, s[3]);
+      q[3] = udot4(x[0], cm[g][3], q[3]);
+      q[6] = udot4(x[1], cm[g][3], q[6]);
+      q[8] = udot4(x[2], cm[g][3], q[8]);
+      q[9] = udot4(x[3], cm[g][3], q[9]);
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < 4; p++) m.s[p] = (int)s[p];
+#pragma unroll
+  for (int i = 0; i < 10; i++) m.q[i] = (int)q[i];
+  return m;
+}
+
 // covar_from_stats (common.rs:334-350), f32 with the reference's order.  The
 // products of sums are integers <= 4080^2 < 2^24 and n is in [1, 16], so
 // x / n is div_rcp with rn = 1.0f / n.
@@ -858,11 +911,13 @@ BCN_HD void enc_mode02(const uint32_t px[16], const Bc7Params& s, int count, int
 // and the reference's key part + bound * 64 orders like (bound, part).
 template <int CH>
 BCN_HD void partition_bounds(const uint32_t px[16], uint16_t* keys, int stride) {
-  Moments full = moments<CH>(px, 0xFFFFu);
+  uint32_t cm[4][4];
+  channel_major(px, cm);
+  Moments full = moments_cm<CH>(cm, 0xFFFFu);
 #pragma unroll 1
   for (int part = 0; part < 64; part++) {
     uint32_t mask = kPart.mask01[part] & 0xFFFFu;
-    Moments m = moments<CH>(px, mask);
+    Moments m = moments_cm<CH>(cm, mask);
     float bound = 0.0f;
 #pragma unroll
     for (int side = 0; side < 2; side++) {
@@ -923,14 +978,14 @@ BCN_HD void partition_bounds(const uint32_t px[16], uint16_t* keys, int stride) 
 // consumed one entry at a time); marks it taken with 0xFFFF, which sorts after
 // every real key.
 BCN_HD int select_next(uint16_t* keys, int stride) {
-  int best = 0x7FFFFFFF, best_i = 0;
+  // key * 64 + i is unique, so its minimum is the first smallest key.
+  uint32_t best = 0xFFFFFFFFu;
 #pragma unroll 8
   for (int i = 0; i < 64; i++) {
-    int key = (int)keys[i * stride] * 64 + i;
-    bool take = key < best;
-    best = take ? key : best;
-    best_i = take ? i : best_i;
+    const uint32_t key = (uint32_t)keys[i * stride] * 64u + (uint32_t)i;
+    best = key < best ? key : best;
   }
+  const int best_i = (int)(best & 63u);
   keys[best_i * stride] = 0xFFFFu;
   return best_i;
 }
