@@ -299,9 +299,39 @@ BCN_HD void bc7_segment(const uint32_t px[16], uint32_t mask, float e0[4], float
 
 // ---- endpoint quantisation (bc7.rs:97-237) ---------------------------------------
 
-BCN_HD int unpack_to_byte(int v, int bits) {
-  int vv = v << (8 - bits);
-  return vv + (vv >> bits);
+// unpack_to_byte (bc7.rs:97-100): vv + (vv >> bits) with vv = v << (8 - bits).
+// For 0 <= v < 2^bits and 4 <= bits <= 8 the two terms share no bits and
+// vv >> bits == v >> (2 * bits - 8), so it is one shift and one shift-or
+// (checked at compile time below).
+BCN_HD constexpr int unpack_to_byte(int v, int bits) {
+  return (v << (8 - bits)) | (v >> (2 * bits - 8));
+}
+
+constexpr bool unpack_to_byte_holds() {
+  for (int bits = 4; bits <= 8; bits++)
+    for (int v = 0; v < (1 << bits); v++) {
+      const int vv = v << (8 - bits);
+      if (unpack_to_byte(v, bits) != vv + (vv >> bits)) return false;
+    }
+  return true;
+}
+static_assert(unpack_to_byte_holds(), "unpack_to_byte as shift-or");
+
+// The two p-bit candidates of one channel code in ep_quant0367 / ep_quant1
+// (bc7.rs:117-123, 159-165): qb_b = 2 * clamp((int)((y - b) / 2 + 0.5), 0,
+// L - 1) + b for y = ep / 255 * (2L - 1), the int conversion truncating and
+// saturating, NaN to 0.
+//   b = 0: y / 2 is exact (or |y| < 2^-125 and the sum is 0.5 either way), so
+//          one fused y * 0.5 + 0.5 rounds like the reference's two steps;
+//   b = 1: for y >= 1/2, y - 1, the halving and the + 0.5 (giving y / 2) are
+//          all exact, so the code is 2 * floor(y / 2) + 1 = floor(y) | 1 with
+//          y clamped to [0, 2L - 1]; y < 1/2 (and NaN) gives 1 either way.
+// Checked over every float with 2^-10 <= |y| in [-4, 2L + 4] (and a sample
+// of smaller ones) against the direct expression in the tests.
+template <int L>
+BCN_HD void pbit_codes(float y, int& qb0, int& qb1) {
+  qb0 = qclamp(__builtin_fmaf(y, 0.5f, 0.5f), 0.0f, (float)(L - 1)) * 2;
+  qb1 = qclamp(y, 0.0f, (float)(2 * L - 1)) | 1;
 }
 
 // ep_quant0367 for one endpoint pair (bc7.rs:102-148) followed by ep_dequant.
@@ -319,15 +349,12 @@ BCN_HD void quant_pair_0367(const float e0[4], const float e1[4], int q0[4], int
     int qb[2][4];
     float eb[2][4];
 #pragma unroll
-    for (int b = 0; b < 2; b++)
+    for (int p = 0; p < QCH; p++) {
+      pbit_codes<L>(div255(ep[p]) * (float)L2, qb[0][p], qb[1][p]);
 #pragma unroll
-      for (int p = 0; p < QCH; p++) {
-        // y / 2 + 0.5 == fma(y, 0.5, 0.5): halving is exact, or |y| < 2^-125
-        // and both round to 0.5.
-        float x = __builtin_fmaf(div255(ep[p]) * (float)L2 - (float)b, 0.5f, 0.5f);
-        qb[b][p] = qclamp(x, 0.0f, (float)(L - 1)) * 2 + b;
+      for (int b = 0; b < 2; b++)
         eb[b][p] = MODE == 0 ? (float)unpack_to_byte(qb[b][p], 5) : (float)qb[b][p];
-      }
+    }
     float err0 = sq(ep[0] - eb[0][0]), err1 = sq(ep[0] - eb[1][0]);
 #pragma unroll
     for (int p = 1; p < ECH; p++) {
@@ -353,16 +380,13 @@ BCN_HD void quant_pair_1(const float e0[4], const float e1[4], int q0[4], int q1
   int qb[2][2][3];
   float eb[2][2][3];
 #pragma unroll
-  for (int b = 0; b < 2; b++)
+  for (int i = 0; i < 2; i++)
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int p = 0; p < 3; p++) {
+      pbit_codes<64>(div255(i == 0 ? e0[p] : e1[p]) * 127.0f, qb[0][i][p], qb[1][i][p]);
 #pragma unroll
-      for (int p = 0; p < 3; p++) {
-        float v = (i == 0 ? e0[p] : e1[p]);
-        float x = __builtin_fmaf(div255(v) * 127.0f - (float)b, 0.5f, 0.5f);  // as in quant_pair_0367
-        qb[b][i][p] = qclamp(x, 0.0f, 63.0f) * 2 + b;
-        eb[b][i][p] = (float)unpack_to_byte(qb[b][i][p], 7);
-      }
+      for (int b = 0; b < 2; b++) eb[b][i][p] = (float)unpack_to_byte(qb[b][i][p], 7);
+    }
   float err0 = sq(e0[0] - eb[0][0][0]), err1 = sq(e0[0] - eb[1][0][0]);
 #pragma unroll
   for (int j = 0; j < 2; j++)

@@ -2,6 +2,7 @@
 // kernels include), so the tests can compare the kernel math with the oracle
 // without a GPU.  Built by tests/kernel_check/build.sh into
 // tests/kernel_check/_build/libkernel_check.so.
+#include <cmath>
 #include <cstring>
 
 #include "../../block_compression_amd/csrc/bc7_block.hpp"
@@ -27,6 +28,50 @@ extern "C" int kcheck_bc7_encode(const void* settings, const uint8_t* rgba, uint
     std::memcpy(out + (size_t)idx * 16, o, 16);
   }
   return 0;
+}
+
+// pbit_codes<L> against the reference's expression, 2 * clamp((int)((y - b)
+// / 2 + 0.5), 0, L - 1) + b with Rust's saturating, NaN-to-0 conversion: every
+// float y with 2^-10 <= |y| in [-4, 2L + 4], every 1024th bit pattern of the
+// smaller magnitudes (all of which give codes 0 and 1), zeros, NaN and the
+// infinities.  Returns the number of mismatches.
+template <int L>
+static long pbit_mismatches() {
+  auto direct = [](float y, int b) {
+    const float x = (y - (float)b) / 2.0f + 0.5f;
+    long v = x != x ? 0 : (x >= 2147483647.0f ? 2147483647L : (x <= -2147483648.0f ? -2147483648L : (long)x));
+    v = v < 0 ? 0 : (v > L - 1 ? L - 1 : v);
+    return (int)(2 * v + b);
+  };
+  long bad = 0;
+  auto check = [&](float y) {
+    int q0, q1;
+    bcn::pbit_codes<L>(y, q0, q1);
+    bad += (q0 != direct(y, 0)) + (q1 != direct(y, 1));
+  };
+  auto bits = [](float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; };
+  auto flt = [](uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; };
+  const uint32_t tiny = bits(0x1p-10f);
+  for (uint32_t u = tiny; u <= bits((float)(2 * L + 4)); u++) check(flt(u));
+  for (uint32_t u = tiny; u <= bits(4.0f); u++) check(-flt(u));
+  for (uint32_t u = 0; u < tiny; u += 1024) {
+    check(flt(u));
+    check(-flt(u));
+  }
+  check(NAN);
+  check(INFINITY);
+  check(-INFINITY);
+  return bad;
+}
+
+extern "C" long kcheck_pbit_mismatches(int levels) {
+  switch (levels) {
+    case 16: return pbit_mismatches<16>();
+    case 32: return pbit_mismatches<32>();
+    case 64: return pbit_mismatches<64>();
+    case 128: return pbit_mismatches<128>();
+    default: return -1;
+  }
 }
 
 #include "../../block_compression_amd/csrc/bc15_block.hpp"

@@ -33,6 +33,12 @@ def bc7_encode(settings: bytes, img: np.ndarray) -> np.ndarray:
     return out
 
 
+def pbit_mismatches(levels: int) -> int:
+    f = lib().kcheck_pbit_mismatches
+    f.restype = ctypes.c_long
+    return f(levels)
+
+
 def bc15_encode(fmt: int, img: np.ndarray) -> np.ndarray:
     img = np.ascontiguousarray(img)
     h, w, _ = img.shape

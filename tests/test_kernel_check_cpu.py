@@ -29,6 +29,16 @@ def test_bc7_kernel_code_matches_oracle(preset):
     assert np.array_equal(kc.bc7_encode(s, img), oracle_lib.compress_rgba8(6, s, img, threads=4))
 
 
+@pytest.mark.parametrize("levels", [16, 32, 64, 128])
+def test_bc7_pbit_codes_match_reference_expression(levels):
+    # The endpoint quantisers' two p-bit candidates (pbit_codes in
+    # bc7_block.hpp) against the reference's float expression of
+    # bc7.rs:117-123 over every float in the range the codes depend on; a
+    # version that ignored the round-to-even tie at y = 1 - 2^-24 failed here
+    # with one mismatch per level count.
+    assert kc.pbit_mismatches(levels) == 0
+
+
 @pytest.mark.parametrize("fmt", [0, 1, 2, 3, 4], ids=["bc1", "bc2", "bc3", "bc4", "bc5"])
 def test_bc15_kernel_code_matches_oracle(fmt):
     img = _alpha_img(128, 128, 9)
