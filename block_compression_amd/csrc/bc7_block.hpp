@@ -437,7 +437,11 @@ BCN_HD void quant_pair(const float e0[4], const float e1[4], int q0[4], int q1[4
 // Index per texel against dequantised (integer) endpoints d[j][0|1][p] of
 // subset j, pattern giving each texel's subset.  Returns the exact SSE.
 // Bounds: |proj|, div <= 4*255^2 and the error sum <= 16*4*255^2 < 2^24.
-template <int CH, int BITS, int NSUB, int ROT = 0>
+// UNIFORM: the caller runs one pattern on the whole wave; a 3-subset texel's
+// data is then one indexed register move per value (s_set_gpr_idx) instead
+// of two selects.  (A pattern that differs across lanes would still give the
+// same results, through a slower loop over the distinct indices.)
+template <int CH, int BITS, int NSUB, int ROT = 0, bool UNIFORM = false>
 BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pattern,
                        uint32_t qblock[2], uint32_t sh = 0) {
   constexpr int L = 1 << BITS;
@@ -462,6 +466,18 @@ BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pat
     // and proj is 0, so the quotient below is NaN either way.
     frcp[j] = rcp_normal(fdiv[j]);
   }
+  uint32_t sd[18];
+  if (UNIFORM && NSUB == 3) {
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      sd[6 * j] = ea_rb[j];
+      sd[6 * j + 1] = ea_ga[j];
+      sd[6 * j + 2] = de_rb[j];
+      sd[6 * j + 3] = de_ga[j];
+      sd[6 * j + 4] = __builtin_bit_cast(uint32_t, fdiv[j]);
+      sd[6 * j + 5] = __builtin_bit_cast(uint32_t, frcp[j]);
+    }
+  }
   const uint32_t ga_mask = CH == 4 ? 0x00FF00FFu : 0x000000FFu;
   const uint32_t m64 = fresh(0xFFC0FFC0u);
   int total = 0;
@@ -472,9 +488,26 @@ BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pat
     int j = NSUB == 1 ? 0 : (int)((pattern >> (2 * k)) & 3u);
     // Subset selects with constant indices keep the arrays in registers.
 #define BCN_PICK(v) (NSUB == 1 ? v[0] : (NSUB == 2 ? (j == 0 ? v[0] : v[1]) : (j == 0 ? v[0] : (j == 1 ? v[1] : v[2]))))
-    const uint32_t erb = BCN_PICK(ea_rb), ega = BCN_PICK(ea_ga);
-    const uint32_t drb = BCN_PICK(de_rb), dga = BCN_PICK(de_ga);
-    const float div = BCN_PICK(fdiv), rdiv = BCN_PICK(frcp);
+    uint32_t erb, ega, drb, dga;
+    float div, rdiv;
+    if (UNIFORM && NSUB == 3) {
+      // One 18-word array, so that the compiler indexes registers instead of
+      // expanding a 3-element extract into selects.
+      const int o = 6 * j;
+      erb = sd[o];
+      ega = sd[o + 1];
+      drb = sd[o + 2];
+      dga = sd[o + 3];
+      div = __builtin_bit_cast(float, sd[o + 4]);
+      rdiv = __builtin_bit_cast(float, sd[o + 5]);
+    } else {
+      erb = BCN_PICK(ea_rb);
+      ega = BCN_PICK(ea_ga);
+      drb = BCN_PICK(de_rb);
+      dga = BCN_PICK(de_ga);
+      div = BCN_PICK(fdiv);
+      rdiv = BCN_PICK(frcp);
+    }
 #undef BCN_PICK
     uint32_t t = texel<ROT>(px, k, sh);
     // b = texel - endpoint 0, per channel.
@@ -906,7 +939,7 @@ BCN_HD void fit_partition_planned(const uint32_t px[16], int part, uint16_t* row
       }
     }
   }
-  f.err = block_quant<3, BITS, 3>(px, d, kPart.subset[part_id], f.qblock);
+  f.err = block_quant<3, BITS, 3, 0, true>(px, d, kPart.subset[part_id], f.qblock);
 }
 
 // bc7_enc_mode02's calls (bc7.rs:612-620): every partition 0..count-1, in
