@@ -66,14 +66,18 @@ BCN_HD uint32_t bc1_fast_quant(const uint32_t px[16], int p0, int p1) {
 // compress_block_bc1_core (bc1_to_5.rs:386-429) -> the two BC1 words.
 BCN_HD void bc1_core(const uint32_t px[16], uint32_t out[2]) {
   // compute_covar_dc (bc1_to_5.rs:157-192).  The channel sums are exact
-  // integers; the covariance accumulates in f32 in texel order.
-  int s[3] = {0, 0, 0};
+  // integers (<= 16 * 255), one v_dot4_u32_u8 per group of four texels; the
+  // covariance accumulates in f32 in texel order.
+  int s[3];
+  {
+    uint32_t cm[4][4], us[3] = {0, 0, 0};
+    channel_major(px, cm);
 #pragma unroll
-  for (int k = 0; k < 16; k++) {
-    uint32_t t = fresh(px[k]);
-    s[0] += chi(t, 0);
-    s[1] += chi(t, 1);
-    s[2] += chi(t, 2);
+    for (int g = 0; g < 4; g++)
+#pragma unroll
+      for (int p = 0; p < 3; p++) us[p] = udot4(cm[g][p], 0x01010101u, us[p]);
+#pragma unroll
+    for (int p = 0; p < 3; p++) s[p] = (int)us[p];
   }
   float dc[3] = {(float)s[0] / 16.0f, (float)s[1] / 16.0f, (float)s[2] / 16.0f};
   float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f, c3 = 0.0f, c4 = 0.0f, c5 = 0.0f;
@@ -149,17 +153,26 @@ BCN_HD void bc1_core(const uint32_t px[16], uint32_t out[2]) {
 #pragma unroll
       for (int p = 0; p < 3; p++) r0[p] = r1[p] = dc[p];
     } else {
-      // Sums of products of small integers: exact (<= 16*3*255).
-      int atb1[3] = {0, 0, 0}, sum_q = 0, sum_qq = 0;
+      // Sums of products of small integers: exact (<= 16*3*255), one
+      // v_dot4_u32_u8 per group of four texels over channel-major bytes and
+      // the group's indices as bytes (x = 3 - q = q ^ 3 for 2-bit q).
+      int atb1[3], sum_q, sum_qq;
+      {
+        uint32_t cm[4][4], ua[3] = {0, 0, 0}, uq = 0, uqq = 0;
+        channel_major(px, cm);
 #pragma unroll
-      for (int k = 0; k < 16; k++) {
-        int q = (int)((bits >> (2 * k)) & 3u);
-        int x = 3 - q;
-        sum_q += q;
-        sum_qq += q * q;
-        uint32_t t = fresh(px[k]);
+        for (int g = 0; g < 4; g++) {
+          const uint32_t b = (bits >> (8 * g)) & 0xFFu;
+          const uint32_t qb = (b & 3u) | ((b & 0xCu) << 6) | ((b & 0x30u) << 12) | ((b & 0xC0u) << 18);
+          uq = udot4(qb, 0x01010101u, uq);
+          uqq = udot4(qb, qb, uqq);
 #pragma unroll
-        for (int p = 0; p < 3; p++) atb1[p] += x * chi(t, p);
+          for (int p = 0; p < 3; p++) ua[p] = udot4(cm[g][p], qb ^ 0x03030303u, ua[p]);
+        }
+#pragma unroll
+        for (int p = 0; p < 3; p++) atb1[p] = (int)ua[p];
+        sum_q = (int)uq;
+        sum_qq = (int)uqq;
       }
       float cxx = (float)(144 - 6 * sum_q + sum_qq);
       float cyy = (float)sum_qq;

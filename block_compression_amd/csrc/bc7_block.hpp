@@ -156,22 +156,6 @@ BCN_HD Moments moments(const uint32_t px[16], uint32_t mask, uint32_t sh = 0) {
   return m;
 }
 
-// The texels as channel-major bytes: cm[g][p] holds channel p of texels
-// 4g .. 4g + 3 (the transpose moments() does per call).
-BCN_HD void channel_major(const uint32_t px[16], uint32_t cm[4][4]) {
-#pragma unroll
-  for (int g = 0; g < 4; g++) {
-    const uint32_t t0 = fresh(px[4 * g]), t1 = fresh(px[4 * g + 1]);
-    const uint32_t t2 = fresh(px[4 * g + 2]), t3 = fresh(px[4 * g + 3]);
-    const uint32_t a = perm_b32(t1, t0, 0x06020400u), b = perm_b32(t1, t0, 0x07030501u);
-    const uint32_t c = perm_b32(t3, t2, 0x06020400u), d = perm_b32(t3, t2, 0x07030501u);
-    cm[g][0] = perm_b32(c, a, 0x05040100u);
-    cm[g][1] = perm_b32(d, b, 0x05040100u);
-    cm[g][2] = perm_b32(c, a, 0x07060302u);
-    cm[g][3] = perm_b32(d, b, 0x07060302u);
-  }
-}
-
 // moments() from channel-major bytes: the masked channel bytes are one AND
 // with the group's byte mask, then one v_dot4_u32_u8 per sum.
 template <int CH>

@@ -294,6 +294,23 @@ BCN_HD uint32_t texel(const uint32_t px[16], int k, uint32_t sh) {
 BCN_HD float chf(uint32_t px, int p) { return (float)((px >> (8 * p)) & 0xFFu); }
 BCN_HD int chi(uint32_t px, int p) { return (int)((px >> (8 * p)) & 0xFFu); }
 
+// The texels as channel-major bytes: cm[g][p] holds channel p of texels
+// 4g .. 4g + 3 (byte i = texel 4g + i), two v_perm_b32 per word, so that a
+// sum over a group of byte products is one v_dot4_u32_u8.
+BCN_HD void channel_major(const uint32_t px[16], uint32_t cm[4][4]) {
+#pragma unroll
+  for (int g = 0; g < 4; g++) {
+    const uint32_t t0 = fresh(px[4 * g]), t1 = fresh(px[4 * g + 1]);
+    const uint32_t t2 = fresh(px[4 * g + 2]), t3 = fresh(px[4 * g + 3]);
+    const uint32_t a = perm_b32(t1, t0, 0x06020400u), b = perm_b32(t1, t0, 0x07030501u);
+    const uint32_t c = perm_b32(t3, t2, 0x06020400u), d = perm_b32(t3, t2, 0x07030501u);
+    cm[g][0] = perm_b32(c, a, 0x05040100u);
+    cm[g][1] = perm_b32(d, b, 0x05040100u);
+    cm[g][2] = perm_b32(c, a, 0x07060302u);
+    cm[g][3] = perm_b32(d, b, 0x07060302u);
+  }
+}
+
 // Index weights of BPTC interpolation (2/3/4-bit: 0,21,43,64 / 0,9,..,64 /
 // 0,4,..,64), one byte per index.
 BCN_HD int weight2(int i) { return (int)((0x402B1500u >> (8 * i)) & 0xFFu); }
