@@ -395,6 +395,23 @@ BCN_HD void bc6h_qbounds(Bc6hState& st, const float span[3]) {
   }
 }
 
+// bc6h_test_mode (bc6h.rs:414-446).  Returns false if the span test skips it.
+template <bool SGN>
+BCN_HD bool bc6h_set_mode(Bc6hState& st, int m, float margin) {
+  float span = bc6h_span(m);
+  if (st.max_span * margin > span) return false;
+  st.epb = bc6h_mode_bits(m);
+  float sp[3] = {span, span, span};
+  if (m >= 10 || m <= 1 || m == 5 || m == 9) {
+    st.mode = m;
+  } else {
+    st.mode = m + st.max_span_idx;
+    if (st.max_span_idx < 3) sp[st.max_span_idx] *= 2.0f;
+  }
+  bc6h_qbounds<SGN>(st, sp);
+  return true;
+}
+
 // The current winner: mode, partition, quantised endpoints (16 bits each,
 // endpoint e channel p at half-word e*3+p), indices.
 struct Bc6hBest {
@@ -417,59 +434,41 @@ BCN_HD int load_qep(const uint32_t w[6], int e, int p) {
   return (int)((w[h >> 1] >> (16 * (h & 1))) & 0xFFFFu);
 }
 
-// bc6h_enc_2p_part_fast + bc6h_enc_2p_list (bc6h.rs:304-388) over the
-// candidate partitions in keys[0..count).
+// bc6h_enc_2p_part_fast (bc6h.rs:304-330) after the two segments: quantise
+// the subsets' PCA endpoints e[j][0|1] for the current mode and index the
+// block.  Returns the error; q and qb are the candidate.
 template <bool SGN>
-BCN_HD void bc6h_enc_2p(const Bc6hBlock& b, const Bc6hParams& s, const Bc6hState& st,
-                        const int* keys, int stride, int count, Bc6hBest& best) {
-  if (count == 0) return;
-  float best_err = INFINITY;
-  int best_part = -1;
-  int bq[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-  uint32_t bqb[2] = {0, 0};
-#pragma unroll 1
-  for (int i = 0; i < count; i++) {
-    int part = keys[i * stride] & 31;
-    uint32_t pattern = kPart.subset[part];
-    int q[4][3], d[2][2][3];
+BCN_HD float bc6h_fit_2p(const Bc6hBlock& b, const Bc6hState& st, int part, const float e[2][2][3],
+                         int q[4][3], uint32_t qb[2]) {
+  int d[2][2][3];
 #pragma unroll
-    for (int j = 0; j < 2; j++) {
-      float e0[3], e1[3];
-      fsegment(b, part_mask(part, j), e0, e1);
-      bc6h_qd<SGN>(st, e0, q[2 * j], d[j][0]);
-      bc6h_qd<SGN>(st, e1, q[2 * j + 1], d[j][1]);
-    }
-    uint32_t qb[2];
-    float err = fblock_quant<3, 2>(b, d, pattern, qb);
-    if (err < best_err) {
-      best_err = err;
-      best_part = part;
-#pragma unroll
-      for (int e = 0; e < 4; e++)
-#pragma unroll
-        for (int p = 0; p < 3; p++) bq[e][p] = q[e][p];
-      bqb[0] = qb[0];
-      bqb[1] = qb[1];
-    }
+  for (int j = 0; j < 2; j++) {
+    bc6h_qd<SGN>(st, e[j][0], q[2 * j], d[j][0]);
+    bc6h_qd<SGN>(st, e[j][1], q[2 * j + 1], d[j][1]);
   }
+  return fblock_quant<3, 2>(b, d, kPart.subset[part], qb);
+}
+
+// The refinement and final comparison of bc6h_enc_2p_list (bc6h.rs:356-388)
+// for the best candidate of its list loop (best_err, best_part, bq, bqb).
+template <bool SGN>
+BCN_HD void bc6h_finish_2p(const Bc6hBlock& b, const Bc6hParams& s, const Bc6hState& st,
+                           float best_err, int best_part, int bq[4][3], uint32_t bqb[2],
+                           Bc6hBest& best) {
 #pragma unroll 1
   for (uint32_t it = 0; it < s.refine_iterations_2p; it++) {
-    int q[4][3], d[2][2][3];
+    float e[2][2][3];
 #pragma unroll
-    for (int j = 0; j < 2; j++) {
-      float e0[3], e1[3];
-      fopt_endpoints<3>(b, bqb, part_mask(best_part, j), e0, e1);
-      bc6h_qd<SGN>(st, e0, q[2 * j], d[j][0]);
-      bc6h_qd<SGN>(st, e1, q[2 * j + 1], d[j][1]);
-    }
+    for (int j = 0; j < 2; j++) fopt_endpoints<3>(b, bqb, part_mask(best_part, j), e[j][0], e[j][1]);
+    int q[4][3];
     uint32_t qb[2];
-    float err = fblock_quant<3, 2>(b, d, kPart.subset[best_part], qb);
+    float err = bc6h_fit_2p<SGN>(b, st, best_part, e, q, qb);
     if (err < best_err) {
       best_err = err;
 #pragma unroll
-      for (int e = 0; e < 4; e++)
+      for (int e2 = 0; e2 < 4; e2++)
 #pragma unroll
-        for (int p = 0; p < 3; p++) bq[e][p] = q[e][p];
+        for (int p = 0; p < 3; p++) bq[e2][p] = q[e2][p];
       bqb[0] = qb[0];
       bqb[1] = qb[1];
     }
@@ -484,12 +483,103 @@ BCN_HD void bc6h_enc_2p(const Bc6hBlock& b, const Bc6hParams& s, const Bc6hState
   }
 }
 
+// bc6h_enc_2p_list (bc6h.rs:326-388) for the current mode over the candidate
+// partitions in keys[0..count), the best candidate kept in registers.
+template <bool SGN>
+BCN_HD void bc6h_enc_2p_list(const Bc6hBlock& b, const Bc6hParams& s, const Bc6hState& st,
+                             const int* keys, int stride, int count, Bc6hBest& best) {
+  if (count == 0) return;
+  float best_err = INFINITY;
+  int best_part = keys[0] & 31;
+  int bq[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  uint32_t bqb[2] = {0, 0};
+#pragma unroll 1
+  for (int i = 0; i < count; i++) {
+    const int part = keys[i * stride] & 31;
+    float e[2][2][3];
+#pragma unroll
+    for (int j = 0; j < 2; j++) fsegment(b, part_mask(part, j), e[j][0], e[j][1]);
+    int q[4][3];
+    uint32_t qb[2];
+    const float err = bc6h_fit_2p<SGN>(b, st, part, e, q, qb);
+    if (err < best_err) {
+      best_err = err;
+      best_part = part;
+#pragma unroll
+      for (int e2 = 0; e2 < 4; e2++)
+#pragma unroll
+        for (int p = 0; p < 3; p++) bq[e2][p] = q[e2][p];
+      bqb[0] = qb[0];
+      bqb[1] = qb[1];
+    }
+  }
+  bc6h_finish_2p<SGN>(b, s, st, best_err, best_part, bq, bqb, best);
+}
+
+// Entries of the per-lane row the block encoder uses: the 32 partition keys,
+// then an error and a partition for each of up to 6 two-subset modes.
+constexpr int kBc6hRow = 32 + 2 * 6;
+
+// Successive bc6h_enc_2p_list calls (bc6h.rs:326-388) for the two-subset
+// modes modes[0..n) (mode numbers as bc6h_set_mode takes them, each passing
+// its span test) over the same candidate list.  A partition's two PCA segments depend only on the
+// block and the subset masks, so the loops are swapped: each candidate's
+// segments are fitted once and quantised for every mode, and each mode's
+// best error and partition (strict <, first in list order, as in the list
+// loop of bc6h_enc_2p_list, bc6h.rs:332-355) are kept in the lane's row
+// after the 32 keys.  Then, in the given order, each mode's winner is fitted
+// again (the same computation, so the same candidate), refined and compared
+// with the block's best (bc6h_finish_2p).
+template <bool SGN>
+BCN_HD void bc6h_enc_2p_modes(const Bc6hBlock& b, const Bc6hParams& s, Bc6hState& st,
+                              const int* modes, int n, int* keys, int stride, int count,
+                              Bc6hBest& best) {
+  if (count == 0) return;
+  int* row = keys + 32 * stride;  // mode i: error bits at row[2i], partition at row[2i + 1]
+#pragma unroll 1
+  for (int m = 0; m < n; m++) {
+    row[(2 * m) * stride] = __builtin_bit_cast(int, INFINITY);
+    row[(2 * m + 1) * stride] = keys[0] & 31;
+  }
+#pragma unroll 1
+  for (int i = 0; i < count; i++) {
+    const int part = keys[i * stride] & 31;
+    float e[2][2][3];
+#pragma unroll
+    for (int j = 0; j < 2; j++) fsegment(b, part_mask(part, j), e[j][0], e[j][1]);
+#pragma unroll 1
+    for (int m = 0; m < n; m++) {
+      bc6h_set_mode<SGN>(st, modes[m], 0.0f);
+      int q[4][3];
+      uint32_t qb[2];
+      const float err = bc6h_fit_2p<SGN>(b, st, part, e, q, qb);
+      if (err < __builtin_bit_cast(float, row[(2 * m) * stride])) {
+        row[(2 * m) * stride] = __builtin_bit_cast(int, err);
+        row[(2 * m + 1) * stride] = part;
+      }
+    }
+  }
+#pragma unroll 1
+  for (int m = 0; m < n; m++) {
+    bc6h_set_mode<SGN>(st, modes[m], 0.0f);
+    const int part = row[(2 * m + 1) * stride];
+    float e[2][2][3];
+#pragma unroll
+    for (int j = 0; j < 2; j++) fsegment(b, part_mask(part, j), e[j][0], e[j][1]);
+    int bq[4][3];
+    uint32_t bqb[2];
+    const float err = bc6h_fit_2p<SGN>(b, st, part, e, bq, bqb);
+    bc6h_finish_2p<SGN>(b, s, st, err, part, bq, bqb, best);
+  }
+}
+
 // bc6h_enc_1p (bc6h.rs:390-412).
+// seg0/seg1: the full block's segment (fsegment(b, 0xFFFF)), which does not
+// depend on the mode, so the caller fits it once for modes 10-13.
 template <bool SGN>
 BCN_HD void bc6h_enc_1p(const Bc6hBlock& b, const Bc6hParams& s, const Bc6hState& st,
-                        Bc6hBest& best) {
-  float e0[3], e1[3];
-  fsegment(b, 0xFFFFu, e0, e1);
+                        const float seg0[3], const float seg1[3], Bc6hBest& best) {
+  float e0[3] = {seg0[0], seg0[1], seg0[2]}, e1[3] = {seg1[0], seg1[1], seg1[2]};
   int q[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
   int d[2][2][3];
   bc6h_qd<SGN>(st, e0, q[0], d[0][0]);
@@ -544,23 +634,6 @@ BCN_HD void bc6h_keys(const Bc6hBlock& b, int* keys, int stride, int count) {
     keys[bi * stride] = tmp;
     last = bv;
   }
-}
-
-// bc6h_test_mode (bc6h.rs:414-446).  Returns false if the span test skips it.
-template <bool SGN>
-BCN_HD bool bc6h_set_mode(Bc6hState& st, int m, float margin) {
-  float span = bc6h_span(m);
-  if (st.max_span * margin > span) return false;
-  st.epb = bc6h_mode_bits(m);
-  float sp[3] = {span, span, span};
-  if (m >= 10 || m <= 1 || m == 5 || m == 9) {
-    st.mode = m;
-  } else {
-    st.mode = m + st.max_span_idx;
-    if (st.max_span_idx < 3) sp[st.max_span_idx] *= 2.0f;
-  }
-  bc6h_qbounds<SGN>(st, sp);
-  return true;
 }
 
 // Bitstream of the winner (bc6h.rs:241-286 with the field-table packer).
@@ -642,7 +715,9 @@ BCN_HD void bc6h_code(const Bc6hBest& best, uint32_t out[4]) {
 
 // compress_bc6h_core (bc6h.rs:821-860) for one block of f16 bit patterns
 // (channel p of texel k at bits[p*16+k]); SGN: the signed (SF16) encoding.
-template <bool SGN>
+// SLOW: the settings' slow_mode is set (the kernels are instantiated per
+// path, each with the registers its own search needs).
+template <bool SGN, bool SLOW>
 BCN_HD void encode_bc6h_block(const uint16_t bits[48], const Bc6hParams& s, int* keys, int stride,
                               uint32_t out[4]) {
   Bc6hBlock b;
@@ -687,25 +762,20 @@ BCN_HD void encode_bc6h_block(const uint16_t bits[48], const Bc6hParams& s, int*
   best.qblock[0] = best.qblock[1] = 0;
 
   const int count = (int)s.fast_skip_threshold;
-  bool keys_ready = false;
-  if (s.slow_mode != 0) {
-    const int two_p[6] = {0, 1, 2, 5, 6, 9};
-#pragma unroll 1
-    for (int i = 0; i < 6; i++) {
-      if (!bc6h_set_mode<SGN>(st, two_p[i], 0.0f)) continue;
-      if (!keys_ready) {
-        bc6h_keys(b, keys, stride, count);
-        keys_ready = true;
-      }
-      bc6h_enc_2p<SGN>(b, s, st, keys, stride, count, best);
-    }
+  float seg0[3], seg1[3];
+  if (SLOW) {
+    // Modes 0, 1, 2, 5, 6, 9 in turn (every span test passes with margin 0).
+    const int modes[6] = {0, 1, 2, 5, 6, 9};
+    bc6h_keys(b, keys, stride, count);
+    bc6h_enc_2p_modes<SGN>(b, s, st, modes, 6, keys, stride, count, best);
+    fsegment(b, 0xFFFFu, seg0, seg1);
 #pragma unroll 1
     for (int m = 10; m < 14; m++)
-      if (bc6h_set_mode<SGN>(st, m, 0.0f)) bc6h_enc_1p<SGN>(b, s, st, best);
+      if (bc6h_set_mode<SGN>(st, m, 0.0f)) bc6h_enc_1p<SGN>(b, s, st, seg0, seg1, best);
   } else {
     if (s.fast_skip_threshold > 0) {
       // The span tests only move the current mode; the last one that passes
-      // is the mode bc6h_enc_2p encodes with (bc6h.rs:836-851).
+      // is the mode bc6h_enc_2p_list encodes with (bc6h.rs:836-851).
       bc6h_set_mode<SGN>(st, 9, 0.0f);
       if (s.fast_mode != 0) bc6h_set_mode<SGN>(st, 1, 1.0f);
       bc6h_set_mode<SGN>(st, 6, 1.0f / 1.2f);
@@ -713,22 +783,30 @@ BCN_HD void encode_bc6h_block(const uint16_t bits[48], const Bc6hParams& s, int*
       bc6h_set_mode<SGN>(st, 0, 1.0f / 1.2f);
       bc6h_set_mode<SGN>(st, 2, 1.0f);
       bc6h_keys(b, keys, stride, count);
-      keys_ready = true;
-      bc6h_enc_2p<SGN>(b, s, st, keys, stride, count, best);
+      bc6h_enc_2p_list<SGN>(b, s, st, keys, stride, count, best);
       if (s.fast_mode == 0 && bc6h_set_mode<SGN>(st, 1, 0.0f))
-        bc6h_enc_2p<SGN>(b, s, st, keys, stride, count, best);
+        bc6h_enc_2p_list<SGN>(b, s, st, keys, stride, count, best);
     }
     bc6h_set_mode<SGN>(st, 10, 0.0f);
     bc6h_set_mode<SGN>(st, 11, 1.0f);
     bc6h_set_mode<SGN>(st, 12, 1.0f);
     bc6h_set_mode<SGN>(st, 13, 1.0f);
-    bc6h_enc_1p<SGN>(b, s, st, best);
+    fsegment(b, 0xFFFFu, seg0, seg1);
+    bc6h_enc_1p<SGN>(b, s, st, seg0, seg1, best);
   }
   if (best.mode < 0) {
     out[0] = out[1] = out[2] = out[3] = 0;
   } else {
     bc6h_code(best, out);
   }
+}
+
+// The two paths chosen at run time (host checks).
+template <bool SGN>
+BCN_HD void encode_bc6h_block(const uint16_t bits[48], const Bc6hParams& s, int* keys, int stride,
+                              uint32_t out[4]) {
+  if (s.slow_mode != 0) encode_bc6h_block<SGN, true>(bits, s, keys, stride, out);
+  else encode_bc6h_block<SGN, false>(bits, s, keys, stride, out);
 }
 
 }  // namespace bcn

@@ -20,20 +20,20 @@ constexpr int kBc6hGroup = 64;
 #define BCN_BC6H_WAVES 2
 #endif
 
-template <bool SGN>
+template <bool SGN, bool SLOW>
 __global__ __launch_bounds__(kBc6hGroup) __attribute__((amdgpu_waves_per_eu(BCN_BC6H_WAVES))) void bc6h_encode_kernel(const uint8_t* __restrict__ src,
                                                                  uint32_t pitch, uint32_t src_format,
                                                                  uint32_t bw, uint32_t nblocks,
                                                                  uint32_t* __restrict__ dst,
                                                                  Bc6hParams s) {
-  __shared__ int keys[32 * kBc6hGroup];
+  __shared__ int keys[kBc6hRow * kBc6hGroup];
   uint32_t idx = blockIdx.x * kBc6hGroup + threadIdx.x;
   if (idx >= nblocks) return;
   uint32_t yy = idx / bw, xx = idx - yy * bw;
   uint16_t bits[48];
   load_block_rgb16f(src, pitch, xx, yy, src_format, bits);
   uint32_t out[4];
-  encode_bc6h_block<SGN>(bits, s, keys + threadIdx.x, kBc6hGroup, out);
+  encode_bc6h_block<SGN, SLOW>(bits, s, keys + threadIdx.x, kBc6hGroup, out);
   uint32_t* o = dst + (size_t)idx * 4;
 #pragma unroll
   for (int i = 0; i < 4; i++) o[i] = out[i];
@@ -50,12 +50,12 @@ hipError_t launch_bc6h(const Task& t, hipStream_t stream) {
   if ((((uintptr_t)t.texture | (uintptr_t)t.pitch) & align) != 0) return hipErrorInvalidValue;
   dim3 grid((nblocks + kBc6hGroup - 1) / kBc6hGroup), block(kBc6hGroup);
   uint32_t* dst = reinterpret_cast<uint32_t*>(t.blocks);
-  if (t.variant == BCN_BC6H_SF16)
-    hipLaunchKernelGGL(bc6h_encode_kernel<true>, grid, block, 0, stream, t.texture, t.pitch,
-                       (uint32_t)t.format, bw, nblocks, dst, s);
-  else
-    hipLaunchKernelGGL(bc6h_encode_kernel<false>, grid, block, 0, stream, t.texture, t.pitch,
-                       (uint32_t)t.format, bw, nblocks, dst, s);
+  // One instantiation per (signedness, slow_mode != 0).
+  auto kernel = t.variant == BCN_BC6H_SF16
+                    ? (s.slow_mode != 0 ? bc6h_encode_kernel<true, true> : bc6h_encode_kernel<true, false>)
+                    : (s.slow_mode != 0 ? bc6h_encode_kernel<false, true> : bc6h_encode_kernel<false, false>);
+  hipLaunchKernelGGL(kernel, grid, block, 0, stream, t.texture, t.pitch, (uint32_t)t.format, bw,
+                     nblocks, dst, s);
   return hipGetLastError();
 }
 

@@ -121,7 +121,7 @@ extern "C" int kcheck_bc6h_encode(const void* settings, const uint16_t* rgba16, 
         const uint16_t* p = rgba16 + (size_t)(yy * 4 + y) * stride + (xx * 4 + x) * 4;
         for (int c = 0; c < 3; c++) bits[c * 16 + y * 4 + x] = p[c];
       }
-    int keys[32];
+    int keys[bcn::kBc6hRow];
     uint32_t o[4];
     if (is_signed)
       bcn::encode_bc6h_block<true>(bits, s, keys, 1, o);
