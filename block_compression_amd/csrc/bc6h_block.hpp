@@ -270,48 +270,57 @@ BCN_HD float fblock_quant(const Bc6hBlock& b, const int d[2][2][3], uint32_t pat
   constexpr int L = 1 << BITS;
   float total = 0.0f;
   qblock[0] = qblock[1] = 0;
-  float fdiv[2], fe[2][2][3];
+  // Per subset: endpoint 0 and the endpoint difference as f32 (the
+  // projection), and as integers the difference and 32 + 64 * endpoint 0:
+  // a + ((w * (b - a) + 32) >> 6) == (w * (b - a) + 32 + 64 * a) >> 6, since
+  // 64 * a is a multiple of 64 and >> is floor division.  Endpoints are
+  // 16-bit (signed: within +-32767), so |w * (b - a)| and |64 * a| are below
+  // 2^23 and the 24-bit multiply-add is exact.
+  float fdiv[2], fe0[2][3], fde[2][3];
+  int ide[2][3], ic0[2][3];
 #pragma unroll
   for (int j = 0; j < NSUB; j++) {
 #pragma unroll
     for (int p = 0; p < 3; p++) {
-      fe[j][0][p] = (float)d[j][0][p];
-      fe[j][1][p] = (float)d[j][1][p];
+      fe0[j][p] = (float)d[j][0][p];
+      fde[j][p] = (float)d[j][1][p] - fe0[j][p];
+      ide[j][p] = d[j][1][p] - d[j][0][p];
+      ic0[j][p] = 32 + 64 * d[j][0][p];
     }
-    float dv = sq(fe[j][1][0] - fe[j][0][0]);
-    dv = dv + sq(fe[j][1][1] - fe[j][0][1]);
-    dv = dv + sq(fe[j][1][2] - fe[j][0][2]);
+    float dv = sq(fde[j][0]);
+    dv = dv + sq(fde[j][1]);
+    dv = dv + sq(fde[j][2]);
     fdiv[j] = dv;
   }
 #pragma unroll
   for (int k = 0; k < 16; k++) {
     int j = NSUB == 1 ? 0 : (int)((pattern >> (2 * k)) & 1u);
-    float ea[3], eb[3];
-    int ia[3], ib[3];
+    float ea[3], de[3];
+    int id[3], ic[3];
 #pragma unroll
     for (int p = 0; p < 3; p++) {
-      ea[p] = j == 0 ? fe[0][0][p] : fe[NSUB - 1][0][p];
-      eb[p] = j == 0 ? fe[0][1][p] : fe[NSUB - 1][1][p];
-      ia[p] = j == 0 ? d[0][0][p] : d[NSUB - 1][0][p];
-      ib[p] = j == 0 ? d[0][1][p] : d[NSUB - 1][1][p];
+      ea[p] = j == 0 ? fe0[0][p] : fe0[NSUB - 1][p];
+      de[p] = j == 0 ? fde[0][p] : fde[NSUB - 1][p];
+      id[p] = j == 0 ? ide[0][p] : ide[NSUB - 1][p];
+      ic[p] = j == 0 ? ic0[0][p] : ic0[NSUB - 1][p];
     }
     float div = j == 0 ? fdiv[0] : fdiv[NSUB - 1];
     // Leading 0.0 dropped: a -0.0 first term only flips the sign of a zero
-    // projection, which rounds to the same index.
-    float proj = (b.blk[k] - ea[0]) * (eb[0] - ea[0]);
-    proj = proj + (b.blk[16 + k] - ea[1]) * (eb[1] - ea[1]);
-    proj = proj + (b.blk[32 + k] - ea[2]) * (eb[2] - ea[2]);
+    // projection, which rounds to the same index.  (eb - ea) is the exact
+    // difference of two integers below 2^17.
+    float proj = (b.blk[k] - ea[0]) * de[0];
+    proj = proj + (b.blk[16 + k] - ea[1]) * de[1];
+    proj = proj + (b.blk[32 + k] - ea[2]) * de[2];
     proj = proj / div;
-    int q1 = qclamp(proj * (float)L + 0.5f, 1.0f, (float)(L - 1));
-    int w0 = weight<BITS>(q1 - 1), w1 = weight<BITS>(q1);
+    // proj * L is exact (a power-of-two scaling), so the fused form rounds
+    // once, as proj * L + 0.5 does.
+    int q1 = qclamp(__builtin_fmaf(proj, (float)L, 0.5f), 1.0f, (float)(L - 1));
+    int w0 = weight_mad<BITS>(q1 - 1), w1 = weight_mad<BITS>(q1);
     float err0 = 0.0f, err1 = 0.0f;
 #pragma unroll
     for (int p = 0; p < 3; p++) {
-      // ((64 - w) * a + w * b + 32) >> 6 == a + ((w * (b - a) + 32) >> 6):
-      // 64 * a is a multiple of 64 and >> is floor division.  Endpoints are
-      // 16-bit, so w * (b - a) is below 2^22 in magnitude.
-      float v0 = (float)(ia[p] + ((mul24(w0, ib[p] - ia[p]) + 32) >> 6));
-      float v1 = (float)(ia[p] + ((mul24(w1, ib[p] - ia[p]) + 32) >> 6));
+      float v0 = (float)((mul24(w0, id[p]) + ic[p]) >> 6);
+      float v1 = (float)((mul24(w1, id[p]) + ic[p]) >> 6);
       err0 = err0 + sq(v0 - b.blk[p * 16 + k]);
       err1 = err1 + sq(v1 - b.blk[p * 16 + k]);
     }
