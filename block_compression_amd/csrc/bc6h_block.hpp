@@ -111,18 +111,21 @@ BCN_HD FStats fstats(const Bc6hBlock& b, uint32_t mask) {
   for (int i = 0; i < 15; i++) st.s[i] = 0.0f;
 #pragma unroll
   for (int k = 0; k < 16; k++) {
-    bool on = (mask >> k) & 1u;
+    // A branch, not a select: where the mask is the same on every lane (the
+    // partition-key loop) the scalar unit skips the texel, elsewhere the
+    // accumulation runs under the execution mask.
+    if (!((mask >> k) & 1u)) continue;
     float r = b.blk[k], g = b.blk[16 + k], bb = b.blk[32 + k];
-    st.s[14] = on ? st.s[14] + 1.0f : st.s[14];
-    st.s[10] = on ? st.s[10] + r : st.s[10];
-    st.s[11] = on ? st.s[11] + g : st.s[11];
-    st.s[12] = on ? st.s[12] + bb : st.s[12];
-    st.s[0] = on ? st.s[0] + r * r : st.s[0];
-    st.s[1] = on ? st.s[1] + r * g : st.s[1];
-    st.s[2] = on ? st.s[2] + r * bb : st.s[2];
-    st.s[4] = on ? st.s[4] + g * g : st.s[4];
-    st.s[5] = on ? st.s[5] + g * bb : st.s[5];
-    st.s[7] = on ? st.s[7] + bb * bb : st.s[7];
+    st.s[14] = st.s[14] + 1.0f;
+    st.s[10] = st.s[10] + r;
+    st.s[11] = st.s[11] + g;
+    st.s[12] = st.s[12] + bb;
+    st.s[0] = st.s[0] + r * r;
+    st.s[1] = st.s[1] + r * g;
+    st.s[2] = st.s[2] + r * bb;
+    st.s[4] = st.s[4] + g * g;
+    st.s[5] = st.s[5] + g * bb;
+    st.s[7] = st.s[7] + bb * bb;
   }
   return st;
 }
@@ -203,12 +206,12 @@ BCN_HD void fsegment(const Bc6hBlock& b, uint32_t mask, float e0[3], float e1[3]
   for (int k = 0; k < 16; k++) {
     // Leading 0.0 of the reference's sum dropped: only a zero's sign can
     // differ, and ext*axis + dc absorbs it.
+    if (!((mask >> k) & 1u)) continue;  // a branch, as in fstats
     float dot = ax[0] * (b.blk[k] - dc[0]);
     dot = dot + ax[1] * (b.blk[16 + k] - dc[1]);
     dot = dot + ax[2] * (b.blk[32 + k] - dc[2]);
-    bool on = (mask >> k) & 1u;
-    ext0 = on ? fminf(ext0, dot) : ext0;
-    ext1 = on ? fmaxf(ext1, dot) : ext1;
+    ext0 = fminf(ext0, dot);
+    ext1 = fmaxf(ext1, dot);
   }
   if (ext1 - ext0 < 1.0f) {
     ext0 -= 0.5f;
