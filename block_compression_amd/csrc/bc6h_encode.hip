@@ -12,16 +12,20 @@ namespace bcn {
 
 constexpr int kBc6hGroup = 64;
 
-// Waves per SIMD.  Unconstrained the block search takes 328 registers
-// (256 VGPRs + 72 AGPRs, one wave); limited to two waves it spills about 120
-// dwords per lane and still runs 20% faster (2048x2048 slow: 2.44 -> 1.92 ms);
-// three waves spill about 450 and run 3% slower than two.
+// Waves per SIMD.  Unconstrained the block search took 328 registers (256
+// VGPRs + 72 AGPRs, one wave); capped at 256 (two waves) it spills about 40
+// dwords per lane and ran 20% faster than one wave.  The slow-mode search
+// (six two-subset modes) runs 4-6% faster again at three waves (168 VGPRs,
+// about 230 spilled); the other presets are 1-4% slower there and keep two.
 #ifndef BCN_BC6H_WAVES
 #define BCN_BC6H_WAVES 2
 #endif
+#ifndef BCN_BC6H_SLOW_WAVES
+#define BCN_BC6H_SLOW_WAVES 3
+#endif
 
 template <bool SGN, bool SLOW>
-__global__ __launch_bounds__(kBc6hGroup) __attribute__((amdgpu_waves_per_eu(BCN_BC6H_WAVES))) void bc6h_encode_kernel(const uint8_t* __restrict__ src,
+__global__ __launch_bounds__(kBc6hGroup) __attribute__((amdgpu_waves_per_eu(SLOW ? BCN_BC6H_SLOW_WAVES : BCN_BC6H_WAVES))) void bc6h_encode_kernel(const uint8_t* __restrict__ src,
                                                                  uint32_t pitch, uint32_t src_format,
                                                                  uint32_t bw, uint32_t nblocks,
                                                                  uint32_t* __restrict__ dst,
