@@ -31,6 +31,7 @@ sys.path.insert(0, ROOT)
 METRIC = "BC7-basic 4096² encode Mpix/s at ref PSNR±0.01dB; batched & grad-refined vs plain"
 PEAK_FP32_TFLOPS = 157.3        # MI355X published FP32 vector peak (packed FMA)
 ISSUE_CEILING_TOPS = 39.3       # 256 CU x 64 lanes x 2.4 GHz, one op per lane per cycle
+PACKED_CEILING_TOPS = 78.6      # the same with two ops per lane per cycle (packed, no FMA)
 PEAK_HBM_GBS = 8000.0
 
 
@@ -537,6 +538,8 @@ def main() -> int:
             "ops_per_block": round(ops_per_block, 1),
             "issue_ceiling_no_fma": ISSUE_CEILING_TOPS,
             "frac_of_issue_ceiling": round(achieved / ISSUE_CEILING_TOPS, 4),
+            "packed_ceiling_no_fma": PACKED_CEILING_TOPS,
+            "frac_of_packed_ceiling": round(achieved / PACKED_CEILING_TOPS, 4),
             "algorithmic_bytes_per_launch": nblocks * 80,
             "hbm_frac": round(nblocks * 80 / (kernel_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 5),
         }
