@@ -115,6 +115,7 @@ struct Moments {
 // and every sum of products over the group is one v_dot4_u32_u8.
 template <int CH, int ROT = 0>
 BCN_HD Moments moments(const uint32_t px[16], uint32_t mask, uint32_t sh = 0) {
+  launder_texels(px);
   Moments m{};
   m.n = __builtin_popcount(mask & 0xFFFFu);
   uint32_t s[4] = {0, 0, 0, 0}, q[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -221,6 +222,7 @@ BCN_HD void covar_from_moments(float c[10], const int s[4], const int q[10], int
 template <int CH, int ROT = 0>
 BCN_HD void pca_segment(const uint32_t px[16], uint32_t mask, float e0[4], float e1[4],
                         uint32_t sh = 0) {
+  launder_texels(px);
   Moments m = moments<CH, ROT>(px, mask, sh);
   float s14 = (float)m.n, rn = rcp_normal(s14);  // n in [1, 16]
   float dc[4];
@@ -428,6 +430,7 @@ BCN_HD void quant_pair(const float e0[4], const float e1[4], int q0[4], int q1[4
 template <int CH, int BITS, int NSUB, int ROT = 0, bool UNIFORM = false>
 BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pattern,
                        uint32_t qblock[2], uint32_t sh = 0) {
+  launder_texels(px);
   constexpr int L = 1 << BITS;
   // Per subset: endpoint 0 and the endpoint difference as (R, B) and (G, A)
   // pairs of 16-bit lanes (A lanes 0 for 3 channels).  Lane values: endpoints
@@ -529,6 +532,7 @@ BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pat
 template <int CH, int BITS, int ROT = 0>
 BCN_HD void opt_endpoints(const uint32_t px[16], const uint32_t qblock[2], uint32_t mask,
                           float e0[4], float e1[4], uint32_t sh = 0) {
+  launder_texels(px);
   constexpr int L1 = (1 << BITS) - 1;
   // Four texels at a time, as in moments(): channel-major bytes of the masked
   // texels, index bytes, and one v_dot4_u32_u8 per sum.  x = L1 - q is
@@ -601,12 +605,13 @@ struct ChannelFit {
 
 // Channel byte at `sh` of texel k: the scalar channel of a mode-4/5 candidate.
 BCN_HD int chan(const uint32_t px[16], int k, uint32_t sh) {
-  return (int)((fresh(px[k]) >> sh) & 0xFFu);
+  return (int)((px[k] >> sh) & 0xFFu);  // px laundered by the caller
 }
 
 // channel_opt_quant (bc7.rs:278-320).
 template <int BITS>
 BCN_HD int channel_quant(const uint32_t px[16], uint32_t sh, int e0, int e1, uint32_t qblock[2]) {
+  launder_texels(px);
   constexpr int L = 1 << BITS;
   // |den| in [0.001, 255.001]
   float den = (float)(e1 - e0) + 0.001f, rden = rcp_normal(den);
@@ -623,7 +628,7 @@ BCN_HD int channel_quant(const uint32_t px[16], uint32_t sh, int e0, int e1, uin
   int total = 0;
 #pragma unroll
   for (int k = 0; k < 16; k++) {
-    const uint32_t cv2 = perm_b32(0u, fresh(px[k]), sel);
+    const uint32_t cv2 = perm_b32(0u, px[k], sel);
     // (float)(texel - e0) as a difference of exact small integers.
     float proj = div_rcp(chf(cv2, 0) - fe0, den, rden);
     // proj * L is exact (|proj| <= 255000, a power-of-two factor), so the
@@ -653,6 +658,7 @@ BCN_HD int channel_qd(float e, int& q) {
 // opt_channel (bc7.rs:239-266) with channel_opt_endpoints (bc7.rs:322-369).
 template <int BITS, int EPBITS>
 BCN_HD ChannelFit opt_channel(const uint32_t px[16], uint32_t sh, uint32_t refine) {
+  launder_texels(px);
   constexpr int L1 = (1 << BITS) - 1;
   int mn = 255, mx = 0;
 #pragma unroll
@@ -673,8 +679,8 @@ BCN_HD ChannelFit opt_channel(const uint32_t px[16], uint32_t sh, uint32_t refin
   uint32_t cw[4], usum = 0;
 #pragma unroll
   for (int g = 0; g < 4; g++) {
-    const uint32_t lo = perm_b32(fresh(px[4 * g + 1]), fresh(px[4 * g]), sel);
-    const uint32_t hi = perm_b32(fresh(px[4 * g + 3]), fresh(px[4 * g + 2]), sel);
+    const uint32_t lo = perm_b32(px[4 * g + 1], px[4 * g], sel);
+    const uint32_t hi = perm_b32(px[4 * g + 3], px[4 * g + 2], sel);
     cw[g] = perm_b32(hi, lo, 0x05040100u);
     usum = udot4(cw[g], 0x01010101u, usum);
   }

@@ -278,13 +278,26 @@ BCN_HD void after(float& x, int dep) {
 #endif
 }
 
+// Makes the block's 16 texel registers opaque in place.  The old values are
+// overwritten, so this costs no instruction (unlike a per-access copy through
+// fresh).  Called at the entry of every function that reads texels, it stops
+// the compiler from hoisting per-channel extractions out of the callers'
+// loops or sharing them between calls, which would keep up to 48 unpacked
+// values live.  The array's owner is a non-const register array.
+BCN_HD void launder_texels(const uint32_t px[16]) {
+  uint32_t* p = const_cast<uint32_t*>(px);
+#pragma unroll
+  for (int k = 0; k < 16; k++) p[k] = fresh(p[k]);
+}
+
 // Texel k of a block as the mode-4/5 candidates see it (bc7.rs:680-696):
 // ROT 0 = as stored; ROT 1 = channel byte at `sh` forced to 255 (3-channel
 // rotation); ROT 2 = channel byte at `sh` replaced by alpha (4-channel).
 // Consumers read only bytes 0..2.
+// The caller has laundered px (launder_texels).
 template <int ROT>
 BCN_HD uint32_t texel(const uint32_t px[16], int k, uint32_t sh) {
-  uint32_t t = fresh(px[k]);
+  uint32_t t = px[k];
   if (ROT == 1) return t | (0xFFu << sh);
   if (ROT == 2) return (t & ~(0xFFu << sh)) | ((t >> 24) << sh);
   return t;
