@@ -16,7 +16,7 @@ constexpr int kBc6hGroup = 64;
 // VGPRs + 72 AGPRs, one wave); capped at 256 (two waves) it spills about 40
 // dwords per lane and ran 20% faster than one wave.  The slow-mode search
 // (six two-subset modes) runs 4-6% faster again at three waves (168 VGPRs,
-// about 230 spilled); the other presets are 1-4% slower there and keep two.
+// 140-230 spilled); the other presets are 1-4% slower there and keep two.
 #ifndef BCN_BC6H_WAVES
 #define BCN_BC6H_WAVES 2
 #endif

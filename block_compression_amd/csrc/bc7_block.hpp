@@ -26,10 +26,11 @@
 #include "bcn_common.hpp"
 
 // Occupancy target of kernels running the BC7 search, in waves per SIMD.
-// Unconstrained, the block search takes about 240 VGPRs (2 waves); at 3 waves
-// it fits 168 VGPRs with a few bytes per lane spilled.  On 4096x4096
-// opaque_basic, 2/3/4 waves ran 7.04/6.91/6.95 ms per launch.  The 8 KiB of
-// LDS per 64-lane workgroup allows 5.
+// Unconstrained, the block search took about 240 VGPRs (2 waves); at 3 waves
+// it fits the 168-VGPR cap (167 used, no spill in the per-format kernel).  On
+// 4096x4096 opaque_basic, 2/3/4 waves ran 7.04/6.91/6.95 ms per launch with
+// an earlier version of the search.  The 8 KiB of LDS per 64-lane workgroup
+// allows 5.
 #ifndef BCN_BC7_WAVES
 #define BCN_BC7_WAVES 3
 #endif
