@@ -59,7 +59,7 @@ __global__ void queue_rewind_kernel(uint32_t* next) {
 
 template <int CLASS>
 __device__ __forceinline__ void queue_encode_chunk(const QueueTask* t, const QueueChunk& ch,
-                                                   uint32_t lane, uint16_t* keys) {
+                                                   uint32_t lane, uint16_t* keys, uint32_t* sdata) {
   const uint32_t variant = t->variant, bw = t->bw, pitch = t->pitch;
   const bool vec = t->vec != 0;
   const uint8_t* tex = t->texture;
@@ -73,9 +73,9 @@ __device__ __forceinline__ void queue_encode_chunk(const QueueTask* t, const Que
     uint32_t out[4] = {0u, 0u, 0u, 0u};
     bool half = false;
     if (CLASS == kQueueBc7Rgba) {
-      encode_bc7_block<4>(px, t->s, keys + lane, kQueueWave, out);
+      encode_bc7_block<4>(px, t->s, keys + lane, kQueueWave, sdata + lane, out);
     } else if (CLASS == kQueueBc7Rgb) {
-      encode_bc7_block<3>(px, t->s, keys + lane, kQueueWave, out);
+      encode_bc7_block<3>(px, t->s, keys + lane, kQueueWave, sdata + lane, out);
     } else if (variant == BCN_BC1) {
       bc1_core(px, out);
       half = true;
@@ -125,6 +125,7 @@ __attribute__((amdgpu_waves_per_eu(CLASS == kQueueBc15 ? 8 : BCN_BC7_WAVES,
     const QueueTask* __restrict__ tasks, const QueueChunk* __restrict__ chunks, uint32_t nchunks,
     uint32_t* __restrict__ next) {
   __shared__ uint16_t keys[CLASS == kQueueBc15 ? 1 : 64 * kQueueWave];
+  __shared__ uint32_t sdata[CLASS == kQueueBc15 ? 1 : 12 * kQueueWave];
   const uint32_t lane = threadIdx.x;
   for (;;) {
     uint32_t ci = 0;
@@ -132,7 +133,7 @@ __attribute__((amdgpu_waves_per_eu(CLASS == kQueueBc15 ? 8 : BCN_BC7_WAVES,
     ci = (uint32_t)__builtin_amdgcn_readfirstlane((int)ci);
     if (ci >= nchunks) break;
     const QueueChunk ch = chunks[ci];
-    queue_encode_chunk<CLASS>(tasks + ch.task, ch, lane, keys);
+    queue_encode_chunk<CLASS>(tasks + ch.task, ch, lane, keys, sdata);
   }
 }
 

@@ -421,6 +421,15 @@ BCN_HD void quant_pair(const float e0[4], const float e1[4], int q0[4], int q1[4
 
 // ---- index selection (common.rs:422-481) ----------------------------------------
 
+// A lane's row of 12 words in LDS (word w at p[w * stride]) holding the two
+// subsets' index-search data of a 2-subset block_quant: a texel then takes
+// its subset's six words with one address computation and three LDS reads
+// instead of six selects.  Empty (p null) for the other subset counts.
+struct SubsetRow {
+  uint32_t* p = nullptr;
+  int stride = 0;
+};
+
 // Index per texel against dequantised (integer) endpoints d[j][0|1][p] of
 // subset j, pattern giving each texel's subset.  Returns the exact SSE.
 // Bounds: |proj|, div <= 4*255^2 and the error sum <= 16*4*255^2 < 2^24.
@@ -428,9 +437,10 @@ BCN_HD void quant_pair(const float e0[4], const float e1[4], int q0[4], int q1[4
 // data is then one indexed register move per value (s_set_gpr_idx) instead
 // of two selects.  (A pattern that differs across lanes would still give the
 // same results, through a slower loop over the distinct indices.)
+// 2 subsets: `row` must be set (SubsetRow).
 template <int CH, int BITS, int NSUB, int ROT = 0, bool UNIFORM = false>
 BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pattern,
-                       uint32_t qblock[2], uint32_t sh = 0) {
+                       uint32_t qblock[2], uint32_t sh = 0, SubsetRow row = SubsetRow()) {
   launder_texels(px);
   constexpr int L = 1 << BITS;
   // Per subset: endpoint 0 and the endpoint difference as (R, B) and (G, A)
@@ -453,6 +463,18 @@ BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pat
     // div in [1, 260100], or 0: then rcp_normal gives NaN instead of +inf,
     // and proj is 0, so the quotient below is NaN either way.
     frcp[j] = rcp_normal(fdiv[j]);
+  }
+  if (NSUB == 2) {
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      uint32_t* r = row.p + 6 * j * row.stride;
+      r[0] = ea_rb[j];
+      r[row.stride] = ea_ga[j];
+      r[2 * row.stride] = de_rb[j];
+      r[3 * row.stride] = de_ga[j];
+      r[4 * row.stride] = __builtin_bit_cast(uint32_t, fdiv[j]);
+      r[5 * row.stride] = __builtin_bit_cast(uint32_t, frcp[j]);
+    }
   }
   uint32_t sd[18];
   if (UNIFORM && NSUB == 3) {
@@ -478,7 +500,15 @@ BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pat
 #define BCN_PICK(v) (NSUB == 1 ? v[0] : (NSUB == 2 ? (j == 0 ? v[0] : v[1]) : (j == 0 ? v[0] : (j == 1 ? v[1] : v[2]))))
     uint32_t erb, ega, drb, dga;
     float div, rdiv;
-    if (UNIFORM && NSUB == 3) {
+    if (NSUB == 2) {
+      const uint32_t* r = row.p + 6 * j * row.stride;
+      erb = r[0];
+      ega = r[row.stride];
+      drb = r[2 * row.stride];
+      dga = r[3 * row.stride];
+      div = __builtin_bit_cast(float, r[4 * row.stride]);
+      rdiv = __builtin_bit_cast(float, r[5 * row.stride]);
+    } else if (UNIFORM && NSUB == 3) {
       // One 18-word array, so that the compiler indexes registers instead of
       // expanding a 3-element extract into selects.
       const int o = 6 * j;
@@ -745,7 +775,7 @@ BCN_HD uint32_t pack4(const int q[4], int ch) {
 // (the tail of bc7_enc_mode01237_part_fast, bc7.rs:535-537).
 template <int MODE>
 BCN_HD void fit_from_segments(const uint32_t px[16], int part_id, const float e[3][2][4],
-                              PartFit& f) {
+                              PartFit& f, SubsetRow row = SubsetRow()) {
   constexpr int PAIRS = (MODE == 0 || MODE == 2) ? 3 : 2;
   constexpr int BITS = (MODE == 0 || MODE == 1) ? 3 : 2;
   constexpr int CH = MODE == 7 ? 4 : 3;
@@ -757,25 +787,25 @@ BCN_HD void fit_from_segments(const uint32_t px[16], int part_id, const float e[
     f.q[j][0] = pack4(q0, CH);
     f.q[j][1] = pack4(q1, CH);
   }
-  f.err = block_quant<CH, BITS, PAIRS>(px, d, kPart.subset[part_id], f.qblock);
+  f.err = block_quant<CH, BITS, PAIRS>(px, d, kPart.subset[part_id], f.qblock, 0u, row);
 }
 
 // bc7_enc_mode01237_part_fast (bc7.rs:517-538) for partition `part_id`
 // (already offset by 64 for 3-subset modes).
 template <int MODE>
-BCN_HD void fit_partition(const uint32_t px[16], int part_id, PartFit& f) {
+BCN_HD void fit_partition(const uint32_t px[16], int part_id, PartFit& f, SubsetRow row = SubsetRow()) {
   constexpr int PAIRS = (MODE == 0 || MODE == 2) ? 3 : 2;
   constexpr int CH = MODE == 7 ? 4 : 3;
   float e[3][2][4];
 #pragma unroll
   for (int j = 0; j < PAIRS; j++) bc7_segment<CH>(px, part_mask(part_id, j), e[j][0], e[j][1]);
-  fit_from_segments<MODE>(px, part_id, e, f);
+  fit_from_segments<MODE>(px, part_id, e, f, row);
 }
 
 // The refinement step of bc7_enc_mode01237 (bc7.rs:571-600).
 template <int MODE>
 BCN_HD void refine_partition(const uint32_t px[16], int part_id, const uint32_t qblock_in[2],
-                             PartFit& f) {
+                             PartFit& f, SubsetRow row = SubsetRow()) {
   constexpr int PAIRS = (MODE == 0 || MODE == 2) ? 3 : 2;
   constexpr int BITS = (MODE == 0 || MODE == 1) ? 3 : 2;
   constexpr int CH = MODE == 7 ? 4 : 3;
@@ -789,7 +819,7 @@ BCN_HD void refine_partition(const uint32_t px[16], int part_id, const uint32_t 
     f.q[j][0] = pack4(q0, CH);
     f.q[j][1] = pack4(q1, CH);
   }
-  f.err = block_quant<CH, BITS, PAIRS>(px, d, kPart.subset[part_id], f.qblock);
+  f.err = block_quant<CH, BITS, PAIRS>(px, d, kPart.subset[part_id], f.qblock, 0u, row);
 }
 
 template <int MODE>
@@ -811,11 +841,11 @@ BCN_HD void store_partfit(Bc7Best& b, const PartFit& f, int err, int part_id, in
 // the best candidate bf of partition part_id.
 template <int MODE>
 BCN_HD void finish_mode01237(const uint32_t px[16], const Bc7Params& s, PartFit bf, int part_id,
-                             int opaque_err, int order, Bc7Best& best) {
+                             int opaque_err, int order, Bc7Best& best, SubsetRow row = SubsetRow()) {
 #pragma unroll 1
   for (uint32_t it = 0; it < s.refine_iterations[MODE]; it++) {
     PartFit f;
-    refine_partition<MODE>(px, part_id, bf.qblock, f);
+    refine_partition<MODE>(px, part_id, bf.qblock, f, row);
     if (f.err < bf.err) bf = f;
   }
   int err = MODE != 7 ? bf.err + opaque_err : bf.err;
@@ -1043,7 +1073,7 @@ BCN_HD int select_next(uint16_t* keys, int stride) {
 // PCA fits (3 channels, its two subset masks) are identical for both modes,
 // so each listed partition is fitted once and quantised for both.
 BCN_HD void enc_mode13(const uint32_t px[16], const Bc7Params& s, uint16_t* keys, int stride,
-                       int opaque_err, Bc7Best& best) {
+                       SubsetRow row, int opaque_err, Bc7Best& best) {
   const int t1 = (int)s.fast_skip_threshold_mode1, t3 = (int)s.fast_skip_threshold_mode3;
   if (t1 == 0 && t3 == 0) return;
   partition_bounds<3>(px, keys, stride);
@@ -1059,7 +1089,7 @@ BCN_HD void enc_mode13(const uint32_t px[16], const Bc7Params& s, uint16_t* keys
     for (int j = 0; j < 2; j++) bc7_segment<3>(px, part_mask(part, j), e[j][0], e[j][1]);
     if (r < t1) {
       PartFit f;
-      fit_from_segments<1>(px, part, e, f);
+      fit_from_segments<1>(px, part, e, f, row);
       if (f.err < b1.err) {
         b1 = f;
         p1 = part;
@@ -1074,21 +1104,21 @@ BCN_HD void enc_mode13(const uint32_t px[16], const Bc7Params& s, uint16_t* keys
 #pragma unroll
           for (int p = 0; p < 3; p++) after(e[j][i][p], b1.err);
       PartFit f;
-      fit_from_segments<3>(px, part, e, f);
+      fit_from_segments<3>(px, part, e, f, row);
       if (f.err < b3.err) {
         b3 = f;
         p3 = part;
       }
     }
   }
-  if (t1 != 0) finish_mode01237<1>(px, s, b1, p1, opaque_err, 2, best);
-  if (t3 != 0) finish_mode01237<3>(px, s, b3, p3, opaque_err, 3, best);
+  if (t1 != 0) finish_mode01237<1>(px, s, b1, p1, opaque_err, 2, best, row);
+  if (t3 != 0) finish_mode01237<3>(px, s, b3, p3, opaque_err, 3, best, row);
 }
 
 // bc7_enc_mode7 (bc7.rs:819-851).
 template <int CH>
 BCN_HD void enc_mode7(const uint32_t px[16], const Bc7Params& s, uint16_t* keys, int stride,
-                      Bc7Best& best) {
+                      SubsetRow row, Bc7Best& best) {
   const int t7 = (int)s.fast_skip_threshold_mode7;
   if (t7 == 0) return;
   partition_bounds<CH>(px, keys, stride);
@@ -1099,13 +1129,13 @@ BCN_HD void enc_mode7(const uint32_t px[16], const Bc7Params& s, uint16_t* keys,
   for (int r = 0; r < t7; r++) {
     int part = select_next(keys, stride);
     PartFit f;
-    fit_partition<7>(px, part, f);
+    fit_partition<7>(px, part, f, row);
     if (f.err < bf.err) {
       bf = f;
       p7 = part;
     }
   }
-  finish_mode01237<7>(px, s, bf, p7, 0, 4, best);
+  finish_mode01237<7>(px, s, bf, p7, 0, 4, best, row);
 }
 
 // bc7_enc_mode45_candidate (bc7.rs:657-733) for rotation `rotation`: the
@@ -1352,10 +1382,12 @@ BCN_HD void code_mode6(const Bc7Best& b, uint32_t out[4]) {
 }
 
 // compress_block_bc7_core (bc7.rs:853-867) for one block.  `keys` is a
-// 64-entry scratch row (16-bit) read and written with `stride`.
+// 64-entry scratch row (16-bit) and `sdata` a 12-word one (SubsetRow), both
+// read and written with `stride`.
 template <int CH>
 BCN_HD void encode_bc7_block(const uint32_t px[16], const Bc7Params& s, uint16_t* keys, int stride,
-                             uint32_t out[4]) {
+                             uint32_t* sdata, uint32_t out[4]) {
+  const SubsetRow row{sdata, stride};
   Bc7Best best;
   best.err = 0x7FFFFFFF;  // stands for the reference's f32::INFINITY
   best.order = 0x7FFFFFFF;
@@ -1386,10 +1418,10 @@ BCN_HD void encode_bc7_block(const uint32_t px[16], const Bc7Params& s, uint16_t
 #pragma unroll
   for (int k = 0; k < 16; k++) t[k] = fresh(t[k]);
   if (s.mode_selection[1] != 0) {
-    enc_mode13(t, s, keys, stride, opaque_err, best);
+    enc_mode13(t, s, keys, stride, row, opaque_err, best);
 #pragma unroll
     for (int k = 0; k < 16; k++) t[k] = fresh(t[k]);
-    enc_mode7<CH>(t, s, keys, stride, best);
+    enc_mode7<CH>(t, s, keys, stride, row, best);
   }
 #pragma unroll
   for (int k = 0; k < 16; k++) t[k] = fresh(t[k]);

@@ -24,13 +24,14 @@ __attribute__((amdgpu_waves_per_eu(BCN_BC7_WAVES, BCN_BC7_WAVES))) void bc7_enco
                                                               uint32_t* __restrict__ dst,
                                                               Bc7Params s) {
   __shared__ uint16_t keys[64 * kBc7Wave];
+  __shared__ uint32_t sdata[12 * kBc7Wave];
   uint32_t idx = blockIdx.x * kBc7Wave + threadIdx.x;
   if (idx >= nblocks) return;
   uint32_t yy = idx / bw, xx = idx - yy * bw;
   uint32_t px[16];
   load_block_rgba8(src, pitch, xx, yy, src_format, VEC, px);
   uint32_t out[4];
-  encode_bc7_block<CH>(px, s, keys + threadIdx.x, kBc7Wave, out);
+  encode_bc7_block<CH>(px, s, keys + threadIdx.x, kBc7Wave, sdata + threadIdx.x, out);
   uint32_t* o = dst + (size_t)idx * 4;
   if (VEC) {
     *reinterpret_cast<uint4*>(o) = make_uint4(out[0], out[1], out[2], out[3]);
