@@ -125,7 +125,7 @@ __attribute__((amdgpu_waves_per_eu(CLASS == kQueueBc15 ? 8 : BCN_BC7_WAVES,
     const QueueTask* __restrict__ tasks, const QueueChunk* __restrict__ chunks, uint32_t nchunks,
     uint32_t* __restrict__ next) {
   __shared__ uint16_t keys[CLASS == kQueueBc15 ? 1 : 64 * kQueueWave];
-  __shared__ uint32_t sdata[CLASS == kQueueBc15 ? 1 : 12 * kQueueWave];
+  __shared__ uint32_t sdata[CLASS == kQueueBc15 ? 1 : 18 * kQueueWave];
   const uint32_t lane = threadIdx.x;
   for (;;) {
     uint32_t ci = 0;

@@ -421,10 +421,10 @@ BCN_HD void quant_pair(const float e0[4], const float e1[4], int q0[4], int q1[4
 
 // ---- index selection (common.rs:422-481) ----------------------------------------
 
-// A lane's row of 12 words in LDS (word w at p[w * stride]) holding the two
-// subsets' index-search data of a 2-subset block_quant: a texel then takes
-// its subset's six words with one address computation and three LDS reads
-// instead of six selects.  Empty (p null) for the other subset counts.
+// A lane's row of 18 words in LDS (word w at p[w * stride]) holding the
+// subsets' index-search data of a 2- or 3-subset block_quant: a texel then
+// takes its subset's six words with one address computation and three LDS
+// reads instead of six (2 subsets) or twelve (3) selects.
 struct SubsetRow {
   uint32_t* p = nullptr;
   int stride = 0;
@@ -433,12 +433,8 @@ struct SubsetRow {
 // Index per texel against dequantised (integer) endpoints d[j][0|1][p] of
 // subset j, pattern giving each texel's subset.  Returns the exact SSE.
 // Bounds: |proj|, div <= 4*255^2 and the error sum <= 16*4*255^2 < 2^24.
-// UNIFORM: the caller runs one pattern on the whole wave; a 3-subset texel's
-// data is then one indexed register move per value (s_set_gpr_idx) instead
-// of two selects.  (A pattern that differs across lanes would still give the
-// same results, through a slower loop over the distinct indices.)
-// 2 subsets: `row` must be set (SubsetRow).
-template <int CH, int BITS, int NSUB, int ROT = 0, bool UNIFORM = false>
+// 2 or 3 subsets: `row` must be set (SubsetRow).
+template <int CH, int BITS, int NSUB, int ROT = 0>
 BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pattern,
                        uint32_t qblock[2], uint32_t sh = 0, SubsetRow row = SubsetRow()) {
   launder_texels(px);
@@ -464,9 +460,9 @@ BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pat
     // and proj is 0, so the quotient below is NaN either way.
     frcp[j] = rcp_normal(fdiv[j]);
   }
-  if (NSUB == 2) {
+  if (NSUB >= 2) {
 #pragma unroll
-    for (int j = 0; j < 2; j++) {
+    for (int j = 0; j < NSUB; j++) {
       uint32_t* r = row.p + 6 * j * row.stride;
       r[0] = ea_rb[j];
       r[row.stride] = ea_ga[j];
@@ -474,18 +470,6 @@ BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pat
       r[3 * row.stride] = de_ga[j];
       r[4 * row.stride] = __builtin_bit_cast(uint32_t, fdiv[j]);
       r[5 * row.stride] = __builtin_bit_cast(uint32_t, frcp[j]);
-    }
-  }
-  uint32_t sd[18];
-  if (UNIFORM && NSUB == 3) {
-#pragma unroll
-    for (int j = 0; j < 3; j++) {
-      sd[6 * j] = ea_rb[j];
-      sd[6 * j + 1] = ea_ga[j];
-      sd[6 * j + 2] = de_rb[j];
-      sd[6 * j + 3] = de_ga[j];
-      sd[6 * j + 4] = __builtin_bit_cast(uint32_t, fdiv[j]);
-      sd[6 * j + 5] = __builtin_bit_cast(uint32_t, frcp[j]);
     }
   }
   const uint32_t ga_mask = CH == 4 ? 0x00FF00FFu : 0x000000FFu;
@@ -500,7 +484,7 @@ BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pat
 #define BCN_PICK(v) (NSUB == 1 ? v[0] : (NSUB == 2 ? (j == 0 ? v[0] : v[1]) : (j == 0 ? v[0] : (j == 1 ? v[1] : v[2]))))
     uint32_t erb, ega, drb, dga;
     float div, rdiv;
-    if (NSUB == 2) {
+    if (NSUB >= 2) {
       const uint32_t* r = row.p + 6 * j * row.stride;
       erb = r[0];
       ega = r[row.stride];
@@ -508,16 +492,6 @@ BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pat
       dga = r[3 * row.stride];
       div = __builtin_bit_cast(float, r[4 * row.stride]);
       rdiv = __builtin_bit_cast(float, r[5 * row.stride]);
-    } else if (UNIFORM && NSUB == 3) {
-      // One 18-word array, so that the compiler indexes registers instead of
-      // expanding a 3-element extract into selects.
-      const int o = 6 * j;
-      erb = sd[o];
-      ega = sd[o + 1];
-      drb = sd[o + 2];
-      dga = sd[o + 3];
-      div = __builtin_bit_cast(float, sd[o + 4]);
-      rdiv = __builtin_bit_cast(float, sd[o + 5]);
     } else {
       erb = BCN_PICK(ea_rb);
       ega = BCN_PICK(ea_ga);
@@ -930,7 +904,7 @@ BCN_HD uint32_t code15_bytes(uint32_t c) {
 // modes dequantise their 5-bit codes with unpack_to_byte(q, 5).
 template <int MODE>
 BCN_HD void fit_partition_planned(const uint32_t px[16], int part, uint16_t* row, int stride,
-                                  PartFit& f) {
+                                  SubsetRow srow, PartFit& f) {
   constexpr int BITS = MODE == 0 ? 3 : 2;
   const int part_id = part + 64;
   int d[3][2][4];
@@ -960,27 +934,27 @@ BCN_HD void fit_partition_planned(const uint32_t px[16], int part, uint16_t* row
       }
     }
   }
-  f.err = block_quant<3, BITS, 3, 0, true>(px, d, kPart.subset[part_id], f.qblock);
+  f.err = block_quant<3, BITS, 3>(px, d, kPart.subset[part_id], f.qblock, 0u, srow);
 }
 
 // bc7_enc_mode02's calls (bc7.rs:612-620): every partition 0..count-1, in
 // order (count is 16 for mode 0 and 64 for mode 2).
 template <int MODE>
 BCN_HD void enc_mode02(const uint32_t px[16], const Bc7Params& s, int count, int opaque_err,
-                       int order, uint16_t* row, int stride, Bc7Best& best) {
+                       int order, uint16_t* row, int stride, SubsetRow srow, Bc7Best& best) {
   PartFit bf;
   bf.err = 0x7FFFFFFF;
   int best_part = 0;
 #pragma unroll 1
   for (int part = 0; part < count; part++) {
     PartFit f;
-    fit_partition_planned<MODE>(px, part, row, stride, f);
+    fit_partition_planned<MODE>(px, part, row, stride, srow, f);
     if (f.err < bf.err) {
       bf = f;
       best_part = part + 64;
     }
   }
-  finish_mode01237<MODE>(px, s, bf, best_part, opaque_err, order, best);
+  finish_mode01237<MODE>(px, s, bf, best_part, opaque_err, order, best, srow);
 }
 
 // PCA split bounds of the 64 two-subset partitions (bc7.rs:629-638,
@@ -1382,7 +1356,7 @@ BCN_HD void code_mode6(const Bc7Best& b, uint32_t out[4]) {
 }
 
 // compress_block_bc7_core (bc7.rs:853-867) for one block.  `keys` is a
-// 64-entry scratch row (16-bit) and `sdata` a 12-word one (SubsetRow), both
+// 64-entry scratch row (16-bit) and `sdata` an 18-word one (SubsetRow), both
 // read and written with `stride`.
 template <int CH>
 BCN_HD void encode_bc7_block(const uint32_t px[16], const Bc7Params& s, uint16_t* keys, int stride,
@@ -1412,8 +1386,8 @@ BCN_HD void encode_bc7_block(const uint32_t px[16], const Bc7Params& s, uint16_t
 #pragma unroll
   for (int k = 0; k < 16; k++) t[k] = fresh(px[k]);
   if (s.mode_selection[0] != 0) {
-    enc_mode02<0>(t, s, 16, opaque_err, 0, keys, stride, best);
-    if (s.skip_mode2 == 0) enc_mode02<2>(t, s, 64, opaque_err, 1, keys, stride, best);
+    enc_mode02<0>(t, s, 16, opaque_err, 0, keys, stride, row, best);
+    if (s.skip_mode2 == 0) enc_mode02<2>(t, s, 64, opaque_err, 1, keys, stride, row, best);
   }
 #pragma unroll
   for (int k = 0; k < 16; k++) t[k] = fresh(t[k]);

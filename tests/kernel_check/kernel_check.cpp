@@ -22,7 +22,7 @@ extern "C" int kcheck_bc7_encode(const void* settings, const uint8_t* rgba, uint
                         ((uint32_t)p[3] << 24);
       }
     uint16_t keys[64];
-    uint32_t sdata[12];
+    uint32_t sdata[18];
     uint32_t o[4];
     if (s.channels == 4) bcn::encode_bc7_block<4>(px, s, keys, 1, sdata, o);
     else bcn::encode_bc7_block<3>(px, s, keys, 1, sdata, o);
