@@ -1221,8 +1221,10 @@ BCN_HD void enc_mode6(const uint32_t px[16], const Bc7Params& s, Bc7Best& best) 
 
 BCN_HD uint32_t qbyte(const uint32_t qep[6], int e, int p) { return (qep[e] >> (8 * p)) & 0xFFu; }
 
-// Index of texel k.
-BCN_HD uint32_t qidx(const uint32_t qb[2], int k) { return (qb[k >> 3] >> (4 * (k & 7))) & 15u; }
+// Index of texel k.  k is a run-time value for the anchor texels: a select
+// rather than qb[k >> 3], so that the winner's record stays in registers (a
+// run-time array index puts the whole record in private memory).
+BCN_HD uint32_t qidx(const uint32_t qb[2], int k) { return ((k < 8 ? qb[0] : qb[1]) >> (4 * (k & 7))) & 15u; }
 
 BCN_HD void write_out(const BitWriter& w, uint32_t out[4]) {
   out[0] = (uint32_t)w.lo;
