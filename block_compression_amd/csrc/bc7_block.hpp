@@ -197,26 +197,49 @@ BCN_HD Moments moments_cm(const uint32_t cm[4][4], uint32_t mask) {
 
 // covar_from_stats (common.rs:334-350), f32 with the reference's order.  The
 // products of sums are integers <= 4080^2 < 2^24 and n is in [1, 16], so
-// x / n is div_rcp with rn = 1.0f / n.
-template <int CH>
-BCN_HD void covar_from_moments(float c[10], const int s[4], const int q[10], int n, float rn) {
-  float s14 = (float)n, s10 = (float)s[0], s11 = (float)s[1], s12 = (float)s[2];
-  c[0] = (float)q[0] - div_rcp(s10 * s10, s14, rn);
-  c[1] = (float)q[1] - div_rcp(s10 * s11, s14, rn);
-  c[2] = (float)q[2] - div_rcp(s10 * s12, s14, rn);
-  c[4] = (float)q[4] - div_rcp(s11 * s11, s14, rn);
-  c[5] = (float)q[5] - div_rcp(s11 * s12, s14, rn);
-  c[7] = (float)q[7] - div_rcp(s12 * s12, s14, rn);
+// x / n is div_rcp with rn = 1.0f / n.  POW2: n is a power of two, rn exact,
+// and x / n is the exact product x * rn.
+template <int CH, bool POW2 = false>
+BCN_HD void covar_from_moments(float c[10], const int s[4], const int q[10], float s14, float rn) {
+  const float s10 = (float)s[0], s11 = (float)s[1], s12 = (float)s[2];
+  auto quo = [&](float x) { return POW2 ? x * rn : div_rcp(x, s14, rn); };
+  c[0] = (float)q[0] - quo(s10 * s10);
+  c[1] = (float)q[1] - quo(s10 * s11);
+  c[2] = (float)q[2] - quo(s10 * s12);
+  c[4] = (float)q[4] - quo(s11 * s11);
+  c[5] = (float)q[5] - quo(s11 * s12);
+  c[7] = (float)q[7] - quo(s12 * s12);
   if (CH == 4) {
     float s13 = (float)s[3];
-    c[3] = (float)q[3] - div_rcp(s10 * s13, s14, rn);
-    c[6] = (float)q[6] - div_rcp(s11 * s13, s14, rn);
-    c[8] = (float)q[8] - div_rcp(s12 * s13, s14, rn);
-    c[9] = (float)q[9] - div_rcp(s13 * s13, s14, rn);
+    c[3] = (float)q[3] - quo(s10 * s13);
+    c[6] = (float)q[6] - quo(s11 * s13);
+    c[8] = (float)q[8] - quo(s12 * s13);
+    c[9] = (float)q[9] - quo(s13 * s13);
   } else {
     c[3] = c[6] = c[8] = c[9] = 0.0f;
   }
 }
+
+// Texel counts of the two subsets of each 2-subset shape as floats, with
+// their reciprocals 1.0f / n (correctly rounded, as rcp_normal gives them):
+// one scalar load per shape, indexed by the loop counter, instead of a
+// conversion and a reciprocal per lane.
+struct PartCounts {
+  float n[64][2], rcp[64][2];
+};
+constexpr PartCounts make_part_counts() {
+  PartCounts t{};
+  for (int i = 0; i < 64; i++) {
+    const int n0 = __builtin_popcount(shape_mask(i, 0));
+    const int c[2] = {n0, 16 - n0};
+    for (int j = 0; j < 2; j++) {
+      t.n[i][j] = (float)c[j];
+      t.rcp[i][j] = 1.0f / (float)c[j];
+    }
+  }
+  return t;
+}
+BCN_TABLE PartCounts kPartCounts = make_part_counts();
 
 // block_segment_core (common.rs:483-518): principal-axis endpoints of the
 // masked texels, before the BC7 clamp.
@@ -230,7 +253,7 @@ BCN_HD void pca_segment(const uint32_t px[16], uint32_t mask, float e0[4], float
 #pragma unroll
   for (int p = 0; p < 4; p++) dc[p] = p < CH ? div_rcp((float)m.s[p], s14, rn) : 0.0f;
   float c[10];
-  covar_from_moments<CH>(c, m.s, m.q, m.n, rn);
+  covar_from_moments<CH>(c, m.s, m.q, s14, rn);
   const float inv_var = 1.0f / (256.0f * 256.0f);
   const float eps = 0.001f * 0.001f;
 #pragma unroll
@@ -989,7 +1012,10 @@ BCN_HD void partition_bounds(const uint32_t px[16], uint16_t* keys, int stride) 
         for (int i = 0; i < 10; i++) q[i] = full.q[i] - m.q[i];
       }
       float c[10];
-      covar_from_moments<CH>(c, sm, q, n, rcp_normal((float)n));  // n in [1, 15]
+      // n in [1, 15] and the same in every lane (the shape is the loop's).
+      const float fn = kPartCounts.n[part][side], rn = kPartCounts.rcp[part][side];
+      if ((n & (n - 1)) == 0) covar_from_moments<CH, true>(c, sm, q, fn, rn);
+      else covar_from_moments<CH>(c, sm, q, fn, rn);
       // get_pca_bound (common.rs:208-245).
       const float inv_var = 1.0f / (256.0f * 256.0f);
       const float eps = 0.001f * 0.001f;

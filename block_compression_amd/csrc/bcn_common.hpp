@@ -346,13 +346,33 @@ BCN_HD int weight_mad(int i) {
   return (mul24(i, K) + 32) >> 6;
 }
 
+// The 2- and 3-bit weight tables as little-endian bytes: indices 0-3 in the
+// low word, 4-7 in the high word.
+template <int BITS>
+constexpr uint32_t weight_bytes(int word) {
+  uint32_t w = 0;
+  constexpr int K = BITS == 2 ? 1365 : 585;
+  for (int b = 0; b < 4; b++) w |= (uint32_t)(((4 * word + b) * K + 32) >> 6) << (8 * b);
+  return w;
+}
+static_assert(weight_bytes<2>(0) == 0x402B1500u && weight_bytes<3>(0) == 0x1B120900u &&
+                  weight_bytes<3>(1) == 0x40372E25u,
+              "byte tables of the 2- and 3-bit weights");
+
 // Weights of indices q - 1 and q (1 <= q < 2^BITS) as the low and high 16-bit
-// lanes: (q - 1) | q << 16 is one 24-bit multiply-add, then the closed form
+// lanes.  2 and 3 bits: one v_perm_b32 gathers bytes q - 1 and q of the byte
+// table into the lanes' low bytes (selector 0x0C gives a zero byte), its
+// selector (q - 1) | 0x0C << 8 | q << 16 | 0x0C << 24 one 24-bit multiply-add.
+// 4 bits: (q - 1) | q << 16 by the multiply-add, then the closed form
 // lane-wise (i * K + 32 <= 4127 fits the lanes).
 template <int BITS>
 BCN_HD uint32_t weight_pair(int q) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  constexpr int K = BITS == 2 ? 1365 : (BITS == 3 ? 585 : 273);
+  if (BITS < 4) {
+    const uint32_t sel = (uint32_t)mul24(q, 0x10001) + 0x0C000BFFu;
+    return perm_b32(weight_bytes<BITS>(BITS == 3), weight_bytes<BITS>(0), sel);
+  }
+  constexpr int K = 273;
   typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
   const u16x2 qq = __builtin_bit_cast(u16x2, (uint32_t)(mul24(q, 0x10001) - 1));
   const u16x2 k = {K, K}, c32 = {32, 32}, c6 = {6, 6};
