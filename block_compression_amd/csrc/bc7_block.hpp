@@ -446,8 +446,8 @@ BCN_HD void quant_pair(const float e0[4], const float e1[4], int q0[4], int q1[4
 
 // A lane's row of 18 words in LDS (word w at p[w * stride]) holding the
 // subsets' index-search data of a 2- or 3-subset block_quant: a texel then
-// takes its subset's six words with one address computation and three LDS
-// reads instead of six (2 subsets) or twelve (3) selects.
+// takes its subset's words with one address computation and three or four
+// LDS reads instead of six to fourteen selects.
 struct SubsetRow {
   uint32_t* p = nullptr;
   int stride = 0;
@@ -462,13 +462,20 @@ BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pat
                        uint32_t qblock[2], uint32_t sh = 0, SubsetRow row = SubsetRow()) {
   launder_texels(px);
   constexpr int L = 1 << BITS;
+  // One and two subsets work from the texel itself: proj = t.de - a.de and
+  // the rounding term (32 + 64 a) - 64 t, with K = a.de and 32 + 64 a kept
+  // per subset (seven words).  Three subsets keep six words per subset in
+  // the 18-word row and work from b = t - a.
+  constexpr bool KFORM = NSUB != 3;
+  constexpr int W = KFORM ? 7 : 6;
   // Per subset: endpoint 0 and the endpoint difference as (R, B) and (G, A)
   // pairs of 16-bit lanes (A lanes 0 for 3 channels).  Lane values: endpoints
   // 0..255, differences and endpoint-minus-texel in [-255, 255],
   // (w*diff + 32) in [-16288, 16352], interpolated minus texel in
   // [-510, 510]: all exact in 16 bits, squares and products in the 32-bit
-  // dot products.
+  // dot products.  32 + 64 a is at most 16352.
   uint32_t ea_rb[3] = {0, 0, 0}, ea_ga[3] = {0, 0, 0}, de_rb[3] = {0, 0, 0}, de_ga[3] = {0, 0, 0};
+  int negk[3] = {0, 0, 0};
   float fdiv[3] = {1.0f, 1.0f, 1.0f}, frcp[3] = {1.0f, 1.0f, 1.0f};
 #pragma unroll
   for (int j = 0; j < NSUB; j++) {
@@ -482,17 +489,24 @@ BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pat
     // div in [1, 260100], or 0: then rcp_normal gives NaN instead of +inf,
     // and proj is 0, so the quotient below is NaN either way.
     frcp[j] = rcp_normal(fdiv[j]);
+    if (KFORM) {
+      // |a.de| <= 4 * 255 * 255.
+      negk[j] = -dot2_16(ea_ga[j], de_ga[j], dot2_16(ea_rb[j], de_rb[j], 0));
+      ea_rb[j] = pk16(32 + 64 * d[j][0][0], 32 + 64 * d[j][0][2]);
+      ea_ga[j] = pk16(32 + 64 * d[j][0][1], 32 + (CH == 4 ? 64 * d[j][0][3] : 0));
+    }
   }
   if (NSUB >= 2) {
 #pragma unroll
     for (int j = 0; j < NSUB; j++) {
-      uint32_t* r = row.p + 6 * j * row.stride;
+      uint32_t* r = row.p + W * j * row.stride;
       r[0] = ea_rb[j];
       r[row.stride] = ea_ga[j];
       r[2 * row.stride] = de_rb[j];
       r[3 * row.stride] = de_ga[j];
       r[4 * row.stride] = __builtin_bit_cast(uint32_t, fdiv[j]);
       r[5 * row.stride] = __builtin_bit_cast(uint32_t, frcp[j]);
+      if (KFORM) r[6 * row.stride] = (uint32_t)negk[j];
     }
   }
   const uint32_t ga_mask = CH == 4 ? 0x00FF00FFu : 0x000000FFu;
@@ -503,43 +517,54 @@ BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pat
 #pragma unroll
   for (int k = 0; k < 16; k++) {
     int j = NSUB == 1 ? 0 : (int)((pattern >> (2 * k)) & 3u);
-    // Subset selects with constant indices keep the arrays in registers.
-#define BCN_PICK(v) (NSUB == 1 ? v[0] : (NSUB == 2 ? (j == 0 ? v[0] : v[1]) : (j == 0 ? v[0] : (j == 1 ? v[1] : v[2]))))
+    // ea: endpoint 0 (six-word form) or 32 + 64 * endpoint 0 (seven-word).
     uint32_t erb, ega, drb, dga;
     float div, rdiv;
+    int nk;
     if (NSUB >= 2) {
-      const uint32_t* r = row.p + 6 * j * row.stride;
+      const uint32_t* r = row.p + W * j * row.stride;
       erb = r[0];
       ega = r[row.stride];
       drb = r[2 * row.stride];
       dga = r[3 * row.stride];
       div = __builtin_bit_cast(float, r[4 * row.stride]);
       rdiv = __builtin_bit_cast(float, r[5 * row.stride]);
+      nk = KFORM ? (int)r[6 * row.stride] : 0;
     } else {
-      erb = BCN_PICK(ea_rb);
-      ega = BCN_PICK(ea_ga);
-      drb = BCN_PICK(de_rb);
-      dga = BCN_PICK(de_ga);
-      div = BCN_PICK(fdiv);
-      rdiv = BCN_PICK(frcp);
+      erb = ea_rb[0];
+      ega = ea_ga[0];
+      drb = de_rb[0];
+      dga = de_ga[0];
+      div = fdiv[0];
+      rdiv = frcp[0];
+      nk = negk[0];
     }
-#undef BCN_PICK
     uint32_t t = texel<ROT>(px, k, sh);
-    // b = texel - endpoint 0, per channel.
-    const uint32_t brb = pk_sub16(t & 0x00FF00FFu, erb), bga = pk_sub16((t >> 8) & ga_mask, ega);
-    // proj = sum (texel - e0) * (e1 - e0).
-    int proj = dot2_16(bga, dga, dot2_16(brb, drb, 0));
+    const uint32_t trb = t & 0x00FF00FFu, tga = (t >> 8) & ga_mask;
+    // proj = sum (texel - e0) * (e1 - e0), an exact integer either way.
+    int proj;
+    uint32_t c_rb, c_ga;
+    // The reference's ((64 - w) * a + w * e1 + 32) >> 6 with e1 = a + diff:
+    // the 64 * a term is a multiple of 64 and >> is floor division, so it
+    // equals a + ((w * diff + 32) >> 6); minus the texel t that is
+    // (w * diff + 32 - 64 * (t - a)) >> 6, with |w * diff|, |64 * (t - a)|
+    // <= 16320; c = 32 - 64 * (t - a) per lane.
+    if (KFORM) {
+      proj = dot2_16(tga, dga, dot2_16(trb, drb, nk));
+      c_rb = pk_mad16(trb, m64, erb);
+      c_ga = pk_mad16(tga, m64, ega);
+    } else {
+      const uint32_t brb = pk_sub16(trb, erb), bga = pk_sub16(tga, ega);
+      proj = dot2_16(bga, dga, dot2_16(brb, drb, 0));
+      c_rb = pk_round_off16(brb, m64);
+      c_ga = pk_round_off16(bga, m64);
+    }
     // |proj| <= |texel - e0| |e1 - e0| <= 510 sqrt(div) (Cauchy-Schwarz).
     float fp = div_rcp((float)proj, div, rdiv);
     // fp * L is exact (a power-of-two multiple of a quotient in [-510, 510]),
     // so the fused form rounds once, as fp * L + 0.5 does.
     int q1 = qclamp(__builtin_fmaf(fp, (float)L, 0.5f), 1.0f, (float)(L - 1));
     const uint32_t wp = weight_pair<BITS>(q1);
-    // The reference's ((64 - w) * a + w * b + 32) >> 6 with b = a + diff: the
-    // 64 * a term is a multiple of 64 and >> is floor division, so it equals
-    // a + ((w * diff + 32) >> 6); minus the texel (= a + b) that is
-    // (w * diff + 32 - 64 * b) >> 6, with |w * diff|, |64 * b| <= 16320.
-    const uint32_t c_rb = pk_round_off16(brb, m64), c_ga = pk_round_off16(bga, m64);
     const uint32_t t0rb = pk_wmad16<0>(wp, drb, c_rb), t0ga = pk_wmad16<0>(wp, dga, c_ga);
     const uint32_t t1rb = pk_wmad16<1>(wp, drb, c_rb), t1ga = pk_wmad16<1>(wp, dga, c_ga);
     // The error sums continue the running total (every partial sum < 2^23),
