@@ -453,6 +453,22 @@ struct SubsetRow {
   int stride = 0;
 };
 
+// The first index candidate of a 2-bit search, clamp((int)(fl(proj / div) *
+// 4 + 0.5), 1, 3) for integer proj and div = |e1 - e0|^2 in [0, 260100], is
+// 1 + [proj >= t2] + [proj >= t3].  With r = 4 proj / div, the two roundings
+// of the float form move r + 0.5 by less than 2^-24 (2|r| + 1) <= 1 / (2 div)
+// (|proj| <= 510 sqrt(div)), and an r + 0.5 that is not an integer is at least
+// 1 / (2 div) from one, while an integer r + 0.5 is computed exactly (proj /
+// div is then a dyadic fraction); so the float form's floor is that of
+// r + 0.5, and r + 0.5 >= m exactly when proj >= ceil((2m - 1) div / 8).
+// div == 0 (proj == 0, a NaN quotient, index 1): thresholds never reached.
+// tests/kernel_check checks both thresholds of every div against the float
+// form.
+BCN_HD void quant4_thresholds(int div, int& t2, int& t3) {
+  t2 = div == 0 ? 0x7FFFFFFF : (3 * div + 7) >> 3;
+  t3 = div == 0 ? 0x7FFFFFFF : (5 * div + 7) >> 3;
+}
+
 // Index per texel against dequantised (integer) endpoints d[j][0|1][p] of
 // subset j, pattern giving each texel's subset.  Returns the exact SSE.
 // Bounds: |proj|, div <= 4*255^2 and the error sum <= 16*4*255^2 < 2^24.
@@ -468,6 +484,8 @@ BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pat
   // the 18-word row and work from b = t - a.
   constexpr bool KFORM = NSUB != 3;
   constexpr int W = KFORM ? 7 : 6;
+  // 2-bit indices: q1 from two integer thresholds on proj (see quant4_thresholds).
+  constexpr bool ITHR = L == 4;
   // Per subset: endpoint 0 and the endpoint difference as (R, B) and (G, A)
   // pairs of 16-bit lanes (A lanes 0 for 3 channels).  Lane values: endpoints
   // 0..255, differences and endpoint-minus-texel in [-255, 255],
@@ -489,6 +507,12 @@ BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pat
     // div in [1, 260100], or 0: then rcp_normal gives NaN instead of +inf,
     // and proj is 0, so the quotient below is NaN either way.
     frcp[j] = rcp_normal(fdiv[j]);
+    if (ITHR) {
+      int t2, t3;
+      quant4_thresholds(div, t2, t3);
+      fdiv[j] = __builtin_bit_cast(float, t2);
+      frcp[j] = __builtin_bit_cast(float, t3);
+    }
     if (KFORM) {
       // |a.de| <= 4 * 255 * 255.
       negk[j] = -dot2_16(ea_ga[j], de_ga[j], dot2_16(ea_rb[j], de_rb[j], 0));
@@ -559,11 +583,16 @@ BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pat
       c_rb = pk_round_off16(brb, m64);
       c_ga = pk_round_off16(bga, m64);
     }
-    // |proj| <= |texel - e0| |e1 - e0| <= 510 sqrt(div) (Cauchy-Schwarz).
-    float fp = div_rcp((float)proj, div, rdiv);
-    // fp * L is exact (a power-of-two multiple of a quotient in [-510, 510]),
-    // so the fused form rounds once, as fp * L + 0.5 does.
-    int q1 = qclamp(__builtin_fmaf(fp, (float)L, 0.5f), 1.0f, (float)(L - 1));
+    int q1;
+    if (ITHR) {
+      q1 = 1 + (int)(proj >= __builtin_bit_cast(int, div)) + (int)(proj >= __builtin_bit_cast(int, rdiv));
+    } else {
+      // |proj| <= |texel - e0| |e1 - e0| <= 510 sqrt(div) (Cauchy-Schwarz).
+      float fp = div_rcp((float)proj, div, rdiv);
+      // fp * L is exact (a power-of-two multiple of a quotient in [-510, 510]),
+      // so the fused form rounds once, as fp * L + 0.5 does.
+      q1 = qclamp(__builtin_fmaf(fp, (float)L, 0.5f), 1.0f, (float)(L - 1));
+    }
     const uint32_t wp = weight_pair<BITS>(q1);
     const uint32_t t0rb = pk_wmad16<0>(wp, drb, c_rb), t0ga = pk_wmad16<0>(wp, dga, c_ga);
     const uint32_t t1rb = pk_wmad16<1>(wp, drb, c_rb), t1ga = pk_wmad16<1>(wp, dga, c_ga);

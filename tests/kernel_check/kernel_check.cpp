@@ -262,3 +262,26 @@ extern "C" void kcheck_bc7_refine(const uint32_t* blocks, const float* target, s
   rp.pbits = reassign >> 16;
   for (size_t b = 0; b < n; b++) sse1[b] = bcn::bc7_refine_block(blocks + 4 * b, target + b * 64, 1, rp, out + 4 * b, sse0 + b);
 }
+
+// quant4_thresholds against the float form of the 2-bit index candidate,
+// floor(fl(fl(proj / div) * 4 + 0.5)) (fused, as the kernels compute it),
+// which is monotone in proj: for every div in [1, 260100] and m = 2, 3 the
+// float form must be below m at proj = t_m - 1 and reach m at t_m.  Returns
+// the number of failures.
+extern "C" long kcheck_quant4_mismatches() {
+  long bad = 0;
+  for (int div = 1; div <= 260100; div++) {
+    int t[2];
+    bcn::quant4_thresholds(div, t[0], t[1]);
+    for (int i = 0; i < 2; i++) {
+      const int m = 2 + i;
+      volatile float below = (float)(t[i] - 1) / (float)div, at = (float)t[i] / (float)div;
+      if (std::floor(std::fma((float)below, 4.0f, 0.5f)) >= (float)m) bad++;
+      if (std::floor(std::fma((float)at, 4.0f, 0.5f)) < (float)m) bad++;
+    }
+  }
+  int t2, t3;
+  bcn::quant4_thresholds(0, t2, t3);
+  if (0 >= t2 || 0 >= t3) bad++;
+  return bad;
+}

@@ -39,6 +39,12 @@ def pbit_mismatches(levels: int) -> int:
     return f(levels)
 
 
+def quant4_mismatches() -> int:
+    f = lib().kcheck_quant4_mismatches
+    f.restype = ctypes.c_long
+    return f()
+
+
 def bc15_encode(fmt: int, img: np.ndarray) -> np.ndarray:
     img = np.ascontiguousarray(img)
     h, w, _ = img.shape

@@ -39,6 +39,14 @@ def test_bc7_pbit_codes_match_reference_expression(levels):
     assert kc.pbit_mismatches(levels) == 0
 
 
+def test_bc7_2bit_index_thresholds_match_float_form():
+    # The 2-bit index searches take their first candidate from two integer
+    # thresholds on the projection (quant4_thresholds in bc7_block.hpp); the
+    # float form they replace is monotone in the projection, so checking both
+    # sides of both thresholds for every divisor 1..260100 covers every input.
+    assert kc.quant4_mismatches() == 0
+
+
 @pytest.mark.parametrize("fmt", [0, 1, 2, 3, 4], ids=["bc1", "bc2", "bc3", "bc4", "bc5"])
 def test_bc15_kernel_code_matches_oracle(fmt):
     img = _alpha_img(128, 128, 9)
