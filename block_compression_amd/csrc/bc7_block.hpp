@@ -469,6 +469,24 @@ BCN_HD void quant4_thresholds(int div, int& t2, int& t3) {
   t3 = div == 0 ? 0x7FFFFFFF : (5 * div + 7) >> 3;
 }
 
+// The first index candidate of a 3-bit search, clamp((int)(fl(proj / div) *
+// 8 + 0.5), 1, 7), is floor(r + 0.5) clamped, r = 8 proj / div (the argument
+// of quant4_thresholds, with L = 8: the float form's error stays below
+// 1 / (2 div) for div <= 260100).  floor(r + 0.5) = floor(N / (4 div)) with
+// N = 32 proj + 2 div + 1, and N / (4 div) (odd over even) is at least
+// 1 / (4 div) from every integer, while fl(N * fl(1 / (4 div))) is within
+// 8.5 * 2^-23 < 1 / (4 * 260100) of it below 8.5 (N is exact: |N| < 2^23).
+// So q1 = clamp(fl(N * c), 1, 7) truncated, c = 1.0f / (4 div); div == 0
+// (proj == 0, index 1) takes c = 0.  tests/kernel_check checks all six
+// thresholds of every div against the float form.
+BCN_HD void quant8_consts(int div, int& c1, float& rq) {
+  c1 = 2 * div + 1;
+  rq = div == 0 ? 0.0f : rcp_normal(4.0f * (float)div);  // 4 div <= 1040400
+}
+BCN_HD int quant8_index(int proj, int c1, float rq) {
+  return qclamp((float)(proj * 32 + c1) * rq, 1.0f, 7.0f);
+}
+
 // Index per texel against dequantised (integer) endpoints d[j][0|1][p] of
 // subset j, pattern giving each texel's subset.  Returns the exact SSE.
 // Bounds: |proj|, div <= 4*255^2 and the error sum <= 16*4*255^2 < 2^24.
@@ -484,8 +502,9 @@ BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pat
   // the 18-word row and work from b = t - a.
   constexpr bool KFORM = NSUB != 3;
   constexpr int W = KFORM ? 7 : 6;
-  // 2-bit indices: q1 from two integer thresholds on proj (see quant4_thresholds).
-  constexpr bool ITHR = L == 4;
+  // 2-bit indices: q1 from two integer thresholds on proj (quant4_thresholds);
+  // 3-bit: from one reciprocal (quant8_index).
+  constexpr bool ITHR = L == 4, IRCP = L == 8;
   // Per subset: endpoint 0 and the endpoint difference as (R, B) and (G, A)
   // pairs of 16-bit lanes (A lanes 0 for 3 channels).  Lane values: endpoints
   // 0..255, differences and endpoint-minus-texel in [-255, 255],
@@ -512,6 +531,11 @@ BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pat
       quant4_thresholds(div, t2, t3);
       fdiv[j] = __builtin_bit_cast(float, t2);
       frcp[j] = __builtin_bit_cast(float, t3);
+    }
+    if (IRCP) {
+      int c1;
+      quant8_consts(div, c1, frcp[j]);
+      fdiv[j] = __builtin_bit_cast(float, c1);
     }
     if (KFORM) {
       // |a.de| <= 4 * 255 * 255.
@@ -586,6 +610,8 @@ BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pat
     int q1;
     if (ITHR) {
       q1 = 1 + (int)(proj >= __builtin_bit_cast(int, div)) + (int)(proj >= __builtin_bit_cast(int, rdiv));
+    } else if (IRCP) {
+      q1 = quant8_index(proj, __builtin_bit_cast(int, div), rdiv);
     } else {
       // |proj| <= |texel - e0| |e1 - e0| <= 510 sqrt(div) (Cauchy-Schwarz).
       float fp = div_rcp((float)proj, div, rdiv);

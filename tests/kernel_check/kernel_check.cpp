@@ -285,3 +285,32 @@ extern "C" long kcheck_quant4_mismatches() {
   if (0 >= t2 || 0 >= t3) bad++;
   return bad;
 }
+
+// quant8_consts / quant8_index against the float form of the 3-bit index
+// candidate, clamp(floor(fl(fl(proj / div) * 8 + 0.5)), 1, 7); both are
+// monotone in proj, so for every div in [1, 260100] and m = 2..7 both must be
+// below m at proj = t - 1 and reach m at t, t = ceil((2m - 1) div / 16).
+// Returns the number of failures.
+extern "C" long kcheck_quant8_mismatches() {
+  long bad = 0;
+  auto flt = [](int proj, int div) {
+    volatile float q = (float)proj / (float)div;
+    const float y = std::floor(std::fma((float)q, 8.0f, 0.5f));
+    return y < 1.0f ? 1 : (y > 7.0f ? 7 : (int)y);
+  };
+  for (int div = 1; div <= 260100; div++) {
+    int c1;
+    float rq;
+    bcn::quant8_consts(div, c1, rq);
+    for (int m = 2; m <= 7; m++) {
+      const int t = ((2 * m - 1) * div + 15) >> 4;
+      if (bcn::quant8_index(t - 1, c1, rq) >= m || bcn::quant8_index(t, c1, rq) < m) bad++;
+      if (flt(t - 1, div) >= m || flt(t, div) < m) bad++;
+    }
+  }
+  int c1;
+  float rq;
+  bcn::quant8_consts(0, c1, rq);
+  if (bcn::quant8_index(0, c1, rq) != 1) bad++;
+  return bad;
+}

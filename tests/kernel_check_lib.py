@@ -45,6 +45,12 @@ def quant4_mismatches() -> int:
     return f()
 
 
+def quant8_mismatches() -> int:
+    f = lib().kcheck_quant8_mismatches
+    f.restype = ctypes.c_long
+    return f()
+
+
 def bc15_encode(fmt: int, img: np.ndarray) -> np.ndarray:
     img = np.ascontiguousarray(img)
     h, w, _ = img.shape

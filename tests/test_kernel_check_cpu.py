@@ -47,6 +47,14 @@ def test_bc7_2bit_index_thresholds_match_float_form():
     assert kc.quant4_mismatches() == 0
 
 
+def test_bc7_3bit_index_reciprocal_matches_float_form():
+    # The 3-bit index searches take their first candidate from one product
+    # with a reciprocal (quant8_index in bc7_block.hpp); both it and the float
+    # form are monotone in the projection, so both sides of all six thresholds
+    # for every divisor 1..260100 cover every input.
+    assert kc.quant8_mismatches() == 0
+
+
 @pytest.mark.parametrize("fmt", [0, 1, 2, 3, 4], ids=["bc1", "bc2", "bc3", "bc4", "bc5"])
 def test_bc15_kernel_code_matches_oracle(fmt):
     img = _alpha_img(128, 128, 9)
