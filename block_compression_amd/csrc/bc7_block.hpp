@@ -241,15 +241,14 @@ constexpr PartCounts make_part_counts() {
 }
 BCN_TABLE PartCounts kPartCounts = make_part_counts();
 
-// block_segment_core (common.rs:483-518): principal-axis endpoints of the
-// masked texels, before the BC7 clamp.
+// block_segment_core (common.rs:483-518), first part: the masked texels'
+// mean dc and principal axis.
 template <int CH, int ROT = 0>
-BCN_HD void pca_segment(const uint32_t px[16], uint32_t mask, float e0[4], float e1[4],
-                        uint32_t sh = 0) {
+BCN_HD void pca_axis(const uint32_t px[16], uint32_t mask, float axis[4], float dc[4],
+                     uint32_t sh = 0) {
   launder_texels(px);
   Moments m = moments<CH, ROT>(px, mask, sh);
   float s14 = (float)m.n, rn = rcp_normal(s14);  // n in [1, 16]
-  float dc[4];
 #pragma unroll
   for (int p = 0; p < 4; p++) dc[p] = p < CH ? div_rcp((float)m.s[p], s14, rn) : 0.0f;
   float c[10];
@@ -262,9 +261,18 @@ BCN_HD void pca_segment(const uint32_t px[16], uint32_t mask, float e0[4], float
   c[4] += eps;
   c[7] += eps;
   c[9] += eps;
-  float axis[4];
   compute_axis<CH>(axis, c, 8);
+}
 
+// The rest of block_segment_core: the masked texels' projections onto the
+// axis give the extent, and the endpoints are the extent's ends (widened to
+// at least 1) on the axis through dc; before the BC7 clamp.
+template <int CH, int ROT = 0>
+BCN_HD void pca_segment(const uint32_t px[16], uint32_t mask, float e0[4], float e1[4],
+                        uint32_t sh = 0) {
+  float axis[4], dc[4];
+  pca_axis<CH, ROT>(px, mask, axis, dc, sh);
+  launder_texels(px);
   float ext0 = INFINITY, ext1 = -INFINITY;
 #pragma unroll
   for (int k = 0; k < 16; k++) {
@@ -1148,6 +1156,75 @@ BCN_HD int select_next(uint16_t* keys, int stride) {
   return best_i;
 }
 
+// fminf / fmaxf into a float in LDS: one non-returning LDS atomic (only the
+// owning lane touches the word, so it is an ordinary read-modify-write).
+BCN_HD void lds_fmin(float* p, float v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+#else
+  *p = fminf(*p, v);
+#endif
+}
+BCN_HD void lds_fmax(float* p, float v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+#else
+  *p = fmaxf(*p, v);
+#endif
+}
+
+// bc7_segment<3> of both subsets of a 2-subset shape whose masks differ
+// between lanes (mask0: subset 0; subset 1 is the rest).  Each texel is
+// projected once, onto its own subset's axis, instead of all 16 texels once
+// per subset under an execution mask: the subsets' axes and means go to the
+// lane's LDS row (words 8j + 0..2 and 8j + 3..5), and each projection,
+// computed in pca_segment's order, updates its subset's extent in words
+// 8j + 6 and 8j + 7 by an LDS min / max.  A minimum or maximum does not
+// depend on the order of its operands (the sign of a zero extent never
+// reaches an output, as in pca_segment), so the endpoints are bc7_segment's.
+BCN_HD void bc7_segment_pair3(const uint32_t px[16], uint32_t mask0, float e[2][2][4], SubsetRow row) {
+  float* r = reinterpret_cast<float*>(row.p);
+  const int st = row.stride;
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    float axis[4], dc[4];
+    pca_axis<3>(px, j == 0 ? mask0 : (~mask0 & 0xFFFFu), axis, dc);
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+      r[(8 * j + p) * st] = axis[p];
+      r[(8 * j + 3 + p) * st] = dc[p];
+    }
+    r[(8 * j + 6) * st] = INFINITY;
+    r[(8 * j + 7) * st] = -INFINITY;
+  }
+  launder_texels(px);
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    float* q = r + (((mask0 >> k) & 1u) ? 0 : 8 * st);
+    const uint32_t t = texel<0>(px, k, 0u);
+    float dot = q[0] * (chf(t, 0) - q[3 * st]);
+    dot = dot + q[st] * (chf(t, 1) - q[4 * st]);
+    dot = dot + q[2 * st] * (chf(t, 2) - q[5 * st]);
+    lds_fmin(q + 6 * st, dot);
+    lds_fmax(q + 7 * st, dot);
+  }
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    const float* q = r + 8 * j * st;
+    float ext0 = q[6 * st], ext1 = q[7 * st];
+    if (ext1 - ext0 < 1.0f) {
+      ext0 -= 0.5f;
+      ext1 += 0.5f;
+    }
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+      const float a = p < 3 ? q[p * st] : 0.0f, d = p < 3 ? q[(3 + p) * st] : 0.0f;
+      e[j][0][p] = p < 3 ? rclamp(ext0 * a + d, 0.0f, 255.0f) : 0.0f;
+      e[j][1][p] = p < 3 ? rclamp(ext1 * a + d, 0.0f, 255.0f) : 0.0f;
+    }
+  }
+}
+
 // bc7_enc_mode13 (bc7.rs:622-655).  Mode 1 takes the t1 smallest keys and
 // mode 3 the t3 smallest, from the same ascending list; a partition's two
 // PCA fits (3 channels, its two subset masks) are identical for both modes,
@@ -1165,8 +1242,7 @@ BCN_HD void enc_mode13(const uint32_t px[16], const Bc7Params& s, uint16_t* keys
   for (int r = 0; r < n; r++) {
     int part = select_next(keys, stride);
     float e[3][2][4];
-#pragma unroll
-    for (int j = 0; j < 2; j++) bc7_segment<3>(px, part_mask(part, j), e[j][0], e[j][1]);
+    bc7_segment_pair3(px, part_mask(part, 0), e, row);
     if (r < t1) {
       PartFit f;
       fit_from_segments<1>(px, part, e, f, row);
