@@ -243,11 +243,8 @@ BCN_TABLE PartCounts kPartCounts = make_part_counts();
 
 // block_segment_core (common.rs:483-518), first part: the masked texels'
 // mean dc and principal axis.
-template <int CH, int ROT = 0>
-BCN_HD void pca_axis(const uint32_t px[16], uint32_t mask, float axis[4], float dc[4],
-                     uint32_t sh = 0) {
-  launder_texels(px);
-  Moments m = moments<CH, ROT>(px, mask, sh);
+template <int CH>
+BCN_HD void pca_axis_moments(const Moments& m, float axis[4], float dc[4]) {
   float s14 = (float)m.n, rn = rcp_normal(s14);  // n in [1, 16]
 #pragma unroll
   for (int p = 0; p < 4; p++) dc[p] = p < CH ? div_rcp((float)m.s[p], s14, rn) : 0.0f;
@@ -262,6 +259,13 @@ BCN_HD void pca_axis(const uint32_t px[16], uint32_t mask, float axis[4], float 
   c[7] += eps;
   c[9] += eps;
   compute_axis<CH>(axis, c, 8);
+}
+
+template <int CH, int ROT = 0>
+BCN_HD void pca_axis(const uint32_t px[16], uint32_t mask, float axis[4], float dc[4],
+                     uint32_t sh = 0) {
+  launder_texels(px);
+  pca_axis_moments<CH>(moments<CH, ROT>(px, mask, sh), axis, dc);
 }
 
 // The rest of block_segment_core: the masked texels' projections onto the
@@ -1185,10 +1189,24 @@ BCN_HD void lds_fmax(float* p, float v) {
 BCN_HD void bc7_segment_pair3(const uint32_t px[16], uint32_t mask0, float e[2][2][4], SubsetRow row) {
   float* r = reinterpret_cast<float*>(row.p);
   const int st = row.stride;
+  // Both subsets' moments from one channel-major copy of the block: subset
+  // 0's masked, subset 1's as the block's minus subset 0's (exact integers).
+  Moments ms[2];
+  {
+    uint32_t cm[4][4];
+    channel_major(px, cm);
+    ms[0] = moments_cm<3>(cm, mask0);
+    const Moments full = moments_cm<3>(cm, 0xFFFFu);
+    ms[1].n = full.n - ms[0].n;
+#pragma unroll
+    for (int i = 0; i < 4; i++) ms[1].s[i] = full.s[i] - ms[0].s[i];
+#pragma unroll
+    for (int i = 0; i < 10; i++) ms[1].q[i] = full.q[i] - ms[0].q[i];
+  }
 #pragma unroll
   for (int j = 0; j < 2; j++) {
     float axis[4], dc[4];
-    pca_axis<3>(px, j == 0 ? mask0 : (~mask0 & 0xFFFFu), axis, dc);
+    pca_axis_moments<3>(ms[j], axis, dc);
 #pragma unroll
     for (int p = 0; p < 3; p++) {
       r[(8 * j + p) * st] = axis[p];
