@@ -272,10 +272,8 @@ BCN_HD void pca_axis(const uint32_t px[16], uint32_t mask, float axis[4], float 
 // axis give the extent, and the endpoints are the extent's ends (widened to
 // at least 1) on the axis through dc; before the BC7 clamp.
 template <int CH, int ROT = 0>
-BCN_HD void pca_segment(const uint32_t px[16], uint32_t mask, float e0[4], float e1[4],
-                        uint32_t sh = 0) {
-  float axis[4], dc[4];
-  pca_axis<CH, ROT>(px, mask, axis, dc, sh);
+BCN_HD void pca_extent(const uint32_t px[16], uint32_t mask, const float axis[4], const float dc[4],
+                       float e0[4], float e1[4], uint32_t sh = 0) {
   launder_texels(px);
   float ext0 = INFINITY, ext1 = -INFINITY;
 #pragma unroll
@@ -305,6 +303,14 @@ BCN_HD void pca_segment(const uint32_t px[16], uint32_t mask, float e0[4], float
     e0[p] = p < CH ? ext0 * axis[p] + dc[p] : 0.0f;
     e1[p] = p < CH ? ext1 * axis[p] + dc[p] : 0.0f;
   }
+}
+
+template <int CH, int ROT = 0>
+BCN_HD void pca_segment(const uint32_t px[16], uint32_t mask, float e0[4], float e1[4],
+                        uint32_t sh = 0) {
+  float axis[4], dc[4];
+  pca_axis<CH, ROT>(px, mask, axis, dc, sh);
+  pca_extent<CH, ROT>(px, mask, axis, dc, e0, e1, sh);
 }
 
 // block_segment (bc7.rs:371-379): PCA endpoints clamped to [0, 255].
@@ -1023,6 +1029,8 @@ BCN_HD void fit_partition_planned(const uint32_t px[16], int part, uint16_t* row
   constexpr int BITS = MODE == 0 ? 3 : 2;
   const int part_id = part + 64;
   int d[3][2][4];
+  uint32_t cm[4][4];
+  channel_major(px, cm);
 #pragma unroll
   for (int j = 0; j < 3; j++) {
     const int op = MODE == 0 ? kPlanMode0.op[part][j] : kPlanMode2.op[part][j];
@@ -1037,8 +1045,16 @@ BCN_HD void fit_partition_planned(const uint32_t px[16], int part, uint16_t* row
         d[j][1][p] = unpack_to_byte((int)((c1 >> (5 * p)) & 0x1Fu), 5);
       }
     } else {
-      float e0[4], e1[4];
-      bc7_segment<3>(px, part_mask(part_id, j), e0, e1);
+      // bc7_segment<3>, its moments from the channel-major copy.
+      const uint32_t mask = part_mask(part_id, j);
+      float e0[4], e1[4], axis[4], dc[4];
+      pca_axis_moments<3>(moments_cm<3>(cm, mask), axis, dc);
+      pca_extent<3>(px, mask, axis, dc, e0, e1);
+#pragma unroll
+      for (int p = 0; p < 3; p++) {
+        e0[p] = rclamp(e0[p], 0.0f, 255.0f);
+        e1[p] = rclamp(e1[p], 0.0f, 255.0f);
+      }
       int q0[4], q1[4];
       quant_pair<MODE, 3>(e0, e1, q0, q1, d[j][0], d[j][1]);
       f.q[j][0] = pack4(q0, 3);
