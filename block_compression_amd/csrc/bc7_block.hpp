@@ -892,8 +892,37 @@ BCN_HD void fit_partition(const uint32_t px[16], int part_id, PartFit& f, Subset
   constexpr int PAIRS = (MODE == 0 || MODE == 2) ? 3 : 2;
   constexpr int CH = MODE == 7 ? 4 : 3;
   float e[3][2][4];
+  if (PAIRS == 2) {
+    // Both subsets' moments from one channel-major copy: subset 0's masked,
+    // subset 1's as the block's minus subset 0's (exact integers).
+    const uint32_t mask0 = part_mask(part_id, 0);
+    Moments ms[2];
+    {
+      uint32_t cm[4][4];
+      channel_major(px, cm);
+      ms[0] = moments_cm<CH>(cm, mask0);
+      const Moments full = moments_cm<CH>(cm, 0xFFFFu);
+      ms[1].n = full.n - ms[0].n;
 #pragma unroll
-  for (int j = 0; j < PAIRS; j++) bc7_segment<CH>(px, part_mask(part_id, j), e[j][0], e[j][1]);
+      for (int i = 0; i < 4; i++) ms[1].s[i] = full.s[i] - ms[0].s[i];
+#pragma unroll
+      for (int i = 0; i < 10; i++) ms[1].q[i] = full.q[i] - ms[0].q[i];
+    }
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      float axis[4], dc[4];
+      pca_axis_moments<CH>(ms[j], axis, dc);
+      pca_extent<CH>(px, j == 0 ? mask0 : (~mask0 & 0xFFFFu), axis, dc, e[j][0], e[j][1]);
+#pragma unroll
+      for (int p = 0; p < CH; p++) {
+        e[j][0][p] = rclamp(e[j][0][p], 0.0f, 255.0f);
+        e[j][1][p] = rclamp(e[j][1][p], 0.0f, 255.0f);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < PAIRS; j++) bc7_segment<CH>(px, part_mask(part_id, j), e[j][0], e[j][1]);
+  }
   fit_from_segments<MODE>(px, part_id, e, f, row);
 }
 
