@@ -150,7 +150,8 @@ struct Bc15 {
   }
 
   // bc1_to_5.rs:323-374
-  void bc1_refine(int32_t pe[2], uint32_t bits, const F dc[3]) const {
+  // `colours` (may be null) receives c0 and c1 before the 565 rounding.
+  void bc1_refine(int32_t pe[2], uint32_t bits, const F dc[3], F* colours = nullptr) const {
     F c0[3], c1[3];
     if ((bits ^ (bits * 4u)) < 4u) {
       for (int p = 0; p < 3; p++) {
@@ -186,6 +187,11 @@ struct Bc15 {
         c1[p] = fclamp(c1[p], F(0.0f), F(255.0f));
       }
     }
+    if (colours)
+      for (int p = 0; p < 3; p++) {
+        colours[p] = c0[p];
+        colours[3 + p] = c1[p];
+      }
     pe[0] = enc_rgb565(c0);
     pe[1] = enc_rgb565(c1);
   }
@@ -357,3 +363,67 @@ void bc5_encode_block(const uint8_t* rgba, size_t stride, uint8_t out[16]) {
 }
 
 }  // namespace bco
+
+using namespace bco;
+
+extern "C" {
+
+// Single primitives of bc1_to_5.rs for the oracle's pin tests.  `block` is
+// the 64-float channel-major block (R[16], G[16], B[16], A[16]).
+
+// enc_rgb565 (bc1_to_5.rs:268-278) of one colour.
+int32_t oracle_bc1_enc_rgb565(float r, float g, float b) {
+  F c[3] = {F(r), F(g), F(b)};
+  return Bc15::enc_rgb565(c);
+}
+
+// dec_rgb565 (bc1_to_5.rs:258-266).
+void oracle_bc1_dec_rgb565(int32_t p, float* rgb3) {
+  F c[3];
+  Bc15::dec_rgb565(c, p);
+  for (int i = 0; i < 3; i++) rgb3[i] = raw(c[i]);
+}
+
+// fast_quant (bc1_to_5.rs:280-321): 2-bit indices of the block's texels
+// against the 565 endpoints p0, p1.
+uint32_t oracle_bc1_fast_quant(const float* block64, int32_t p0, int32_t p1) {
+  Bc15 c;
+  for (int i = 0; i < 64; i++) c.block[i] = F(block64[i]);
+  return c.fast_quant(p0, p1);
+}
+
+// bc1_refine (bc1_to_5.rs:323-374): the least-squares endpoint colours c0,
+// c1 before the 565 rounding (clamped to 0..255), and their 565 codes.
+void oracle_bc1_refine(const float* block64, uint32_t bits, float* c0c1_6, int32_t* pe2) {
+  Bc15 c;
+  for (int i = 0; i < 64; i++) c.block[i] = F(block64[i]);
+  F covar[6], dc[3];
+  c.compute_covar_dc(covar, dc);
+  int32_t pe[2];
+  F col[6];
+  c.bc1_refine(pe, bits, dc, col);
+  pe2[0] = pe[0];
+  pe2[1] = pe[1];
+  if (c0c1_6)
+    for (int i = 0; i < 6; i++) c0c1_6[i] = raw(col[i]);
+}
+
+// The BC1 principal axis: compute_covar_dc + eps + compute_axis3 with 4
+// iterations (bc1_to_5.rs:386-397); also returns the mean colour.
+void oracle_bc1_axis(const float* block64, float* axis3, float* dc3) {
+  Bc15 c;
+  for (int i = 0; i < 64; i++) c.block[i] = F(block64[i]);
+  F covar[6], dc[3], axis[3] = {0.0f, 0.0f, 0.0f};
+  c.compute_covar_dc(covar, dc);
+  const F EPS = F(1.1920928955078125e-07f);
+  covar[0] += EPS;
+  covar[3] += EPS;
+  covar[5] += EPS;
+  Bc15::compute_axis3(axis, covar, 4);
+  for (int p = 0; p < 3; p++) {
+    axis3[p] = raw(axis[p]);
+    dc3[p] = raw(dc[p]);
+  }
+}
+
+}  // extern "C"

@@ -610,3 +610,47 @@ float bc7_encode_block(const uint8_t* rgba, size_t stride, const BC7Settings& s,
 }
 
 }  // namespace bco
+
+using namespace bco;
+
+extern "C" {
+
+// Single primitives of bc7.rs for the oracle's pin tests, in plain floats.
+// One endpoint pair (8 floats: endpoint A RGBA, endpoint B RGBA) through the
+// mode's quantiser: ep_quant0367 (modes 0, 3, 6, 7), ep_quant1 (mode 1) or
+// ep_quant245 (modes 2, 4, 5) (bc7.rs:102-189).
+void oracle_bc7_ep_quant_pair(int mode, const float* ep8, int channels, int32_t* qep8) {
+  F e[8];
+  for (int i = 0; i < 8; i++) e[i] = F(ep8[i]);
+  for (int i = 0; i < 8; i++) qep8[i] = 0;
+  if (mode == 0 || mode == 3 || mode == 6 || mode == 7) ep_quant0367(qep8, e, (size_t)mode, (size_t)channels);
+  else if (mode == 1) ep_quant1(qep8, e);
+  else ep_quant245(qep8, e, (size_t)mode);
+}
+
+// ep_dequant of one pair (bc7.rs:210-232).
+void oracle_bc7_ep_dequant_pair(int mode, const int32_t* qep8, float* ep8) {
+  F e[8];
+  ep_dequant(e, qep8, (size_t)mode);
+  for (int i = 0; i < 8; i++) ep8[i] = raw(e[i]);
+}
+
+// channel_opt_endpoints (bc7.rs:322-369): scalar-channel least squares.
+void oracle_bc7_channel_opt_endpoints(const float* channel16, uint32_t bits, const uint32_t* qblock2,
+                                      float* ep2) {
+  F c[16], e[2] = {0.0f, 0.0f};
+  for (int i = 0; i < 16; i++) c[i] = F(channel16[i]);
+  channel_opt_endpoints(e, c, bits, qblock2);
+  ep2[0] = raw(e[0]);
+  ep2[1] = raw(e[1]);
+}
+
+// channel_opt_quant (bc7.rs:278-320).
+float oracle_bc7_channel_opt_quant(const float* channel16, uint32_t bits, const float* ep2,
+                                   uint32_t* qblock2) {
+  F c[16], e[2] = {F(ep2[0]), F(ep2[1])};
+  for (int i = 0; i < 16; i++) c[i] = F(channel16[i]);
+  return raw(channel_opt_quant(qblock2, c, bits, e));
+}
+
+}  // extern "C"

@@ -305,6 +305,68 @@ void oracle_get_partition_sets(uint8_t* out /* 2*64*16 */) {
 uint16_t oracle_f32_to_f16(float x) { return f32_to_f16_bits(x); }
 float oracle_f16_to_f32(uint16_t h) { return f16_bits_to_f32(h); }
 
+// Shared encoder primitives of common.rs, one call each, in plain floats.
+// `block` is the 64-float channel-major block (R[16], G[16], B[16], A[16]).
+void oracle_compute_axis(const float* covar10, uint32_t iterations, uint32_t channels, float* axis4) {
+  F c[10], a[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int i = 0; i < 10; i++) c[i] = F(covar10[i]);
+  compute_axis(a, c, iterations, channels);
+  for (int i = 0; i < 4; i++) axis4[i] = raw(a[i]);
+}
+float oracle_get_pca_bound(const float* covar10, uint32_t channels) {
+  F c[10];
+  for (int i = 0; i < 10; i++) c[i] = F(covar10[i]);
+  return raw(get_pca_bound(c, channels));
+}
+void oracle_block_pca_axis(const float* block64, uint32_t mask, uint32_t channels, float* axis4,
+                           float* dc4) {
+  F b[64], a[4] = {0.0f, 0.0f, 0.0f, 0.0f}, d[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int i = 0; i < 64; i++) b[i] = F(block64[i]);
+  block_pca_axis(a, d, b, mask, channels);
+  for (int i = 0; i < 4; i++) {
+    axis4[i] = raw(a[i]);
+    dc4[i] = raw(d[i]);
+  }
+}
+void oracle_block_segment_core(const float* block64, uint32_t mask, uint32_t channels, float* ep8) {
+  F b[64], e[8];
+  for (int i = 0; i < 64; i++) b[i] = F(block64[i]);
+  for (int i = 0; i < 8; i++) e[i] = 0.0f;
+  block_segment_core(e, b, mask, channels);
+  for (int i = 0; i < 8; i++) ep8[i] = raw(e[i]);
+}
+void oracle_opt_endpoints(const float* block64, uint32_t bits, const uint32_t* qblock2,
+                          uint32_t mask, uint32_t channels, float* ep8) {
+  F b[64], e[8];
+  for (int i = 0; i < 64; i++) b[i] = F(block64[i]);
+  for (int i = 0; i < 8; i++) e[i] = 0.0f;
+  opt_endpoints(e, b, bits, qblock2, mask, channels);
+  for (int i = 0; i < 8; i++) ep8[i] = raw(e[i]);
+}
+// block_quant over one subset: ep8 holds endpoints A (0..3) and B (4..7) of
+// subset 0; pattern 0 sends every texel to subset 0.
+float oracle_block_quant(const float* block64, uint32_t bits, const float* ep8, uint32_t channels,
+                         uint32_t* qblock2) {
+  F b[64], e[24];
+  for (int i = 0; i < 64; i++) b[i] = F(block64[i]);
+  for (int i = 0; i < 24; i++) e[i] = i < 8 ? F(ep8[i]) : F(0.0f);
+  return raw(block_quant(qblock2, b, bits, e, 0, channels));
+}
+void oracle_compute_stats_masked(const float* block64, uint32_t mask, uint32_t channels, float* stats15) {
+  F b[64], s[15];
+  for (int i = 0; i < 64; i++) b[i] = F(block64[i]);
+  for (int i = 0; i < 15; i++) s[i] = 0.0f;
+  compute_stats_masked(s, b, mask, channels);
+  for (int i = 0; i < 15; i++) stats15[i] = raw(s[i]);
+}
+float oracle_block_pca_bound_split(const float* block64, uint32_t mask, uint32_t channels) {
+  F b[64], s[15];
+  for (int i = 0; i < 64; i++) b[i] = F(block64[i]);
+  for (int i = 0; i < 15; i++) s[i] = 0.0f;
+  compute_stats_masked(s, b, 0xFFFFFFFFu, channels);
+  return raw(block_pca_bound_split(b, mask, s, channels));
+}
+
 #ifdef BC_OPCOUNT
 // Counted op total of encoding the listed blocks (single thread).
 uint64_t oracle_count_ops_rgba8(int variant, const void* settings, const uint8_t* rgba,

@@ -14,7 +14,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
-LIB_PATH = os.path.join(ORACLE_DIR, "_build", "liboracle_bc.so")
+# BCO_ORACLE_LIB names another build of the oracle's sources (the planted-slip
+# builds of test_oracle_pins_catch_slips.py); default: the oracle itself.
+LIB_PATH = os.environ.get("BCO_ORACLE_LIB") or os.path.join(ORACLE_DIR, "_build", "liboracle_bc.so")
 OPCOUNT_PATH = os.path.join(ORACLE_DIR, "_build", "liboracle_bc_opcount.so")
 
 _lib = None
@@ -64,6 +66,30 @@ def _declare(lib):
     lib.oracle_f32_to_f16.restype = ctypes.c_uint16
     lib.oracle_f16_to_f32.argtypes = [ctypes.c_uint16]
     lib.oracle_f16_to_f32.restype = ctypes.c_float
+    # Single encoder primitives (pin tests).
+    lib.oracle_compute_axis.argtypes = [_vp, _u32, _u32, _vp]
+    lib.oracle_get_pca_bound.argtypes = [_vp, _u32]
+    lib.oracle_get_pca_bound.restype = ctypes.c_float
+    lib.oracle_block_pca_axis.argtypes = [_vp, _u32, _u32, _vp, _vp]
+    lib.oracle_block_segment_core.argtypes = [_vp, _u32, _u32, _vp]
+    lib.oracle_opt_endpoints.argtypes = [_vp, _u32, _vp, _u32, _u32, _vp]
+    lib.oracle_block_quant.argtypes = [_vp, _u32, _vp, _u32, _vp]
+    lib.oracle_block_quant.restype = ctypes.c_float
+    lib.oracle_compute_stats_masked.argtypes = [_vp, _u32, _u32, _vp]
+    lib.oracle_block_pca_bound_split.argtypes = [_vp, _u32, _u32]
+    lib.oracle_block_pca_bound_split.restype = ctypes.c_float
+    lib.oracle_bc7_ep_quant_pair.argtypes = [_i32, _vp, _i32, _vp]
+    lib.oracle_bc7_ep_dequant_pair.argtypes = [_i32, _vp, _vp]
+    lib.oracle_bc7_channel_opt_endpoints.argtypes = [_vp, _u32, _vp, _vp]
+    lib.oracle_bc7_channel_opt_quant.argtypes = [_vp, _u32, _vp, _vp]
+    lib.oracle_bc7_channel_opt_quant.restype = ctypes.c_float
+    lib.oracle_bc1_enc_rgb565.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float]
+    lib.oracle_bc1_enc_rgb565.restype = ctypes.c_int32
+    lib.oracle_bc1_dec_rgb565.argtypes = [ctypes.c_int32, _vp]
+    lib.oracle_bc1_fast_quant.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32]
+    lib.oracle_bc1_fast_quant.restype = ctypes.c_uint32
+    lib.oracle_bc1_refine.argtypes = [_vp, _u32, _vp, _vp]
+    lib.oracle_bc1_axis.argtypes = [_vp, _vp, _vp]
 
 
 def lib():
@@ -287,3 +313,137 @@ def count_ops_rgba16(settings: bytes, rgba16: np.ndarray, block_ids: np.ndarray)
     ids = np.ascontiguousarray(block_ids, dtype=np.int64)
     sbuf = ctypes.create_string_buffer(settings)
     return int(oplib().oracle_count_ops_rgba16(sbuf, _ptr(bits), w, w * 4, _ptr(ids), ids.size))
+
+
+# ---- single encoder primitives (the oracle's pin tests) ---------------------
+#
+# `block` arguments are 64 floats, channel-major: R[16], G[16], B[16], A[16].
+
+def _f32(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if n is not None:
+        assert a.size == n
+    return a
+
+
+def compute_axis(covar10, iterations: int, channels: int) -> np.ndarray:
+    """compute_axis (common.rs:260-292)."""
+    c = _f32(covar10, 10)
+    out = np.zeros(4, np.float32)
+    lib().oracle_compute_axis(_ptr(c), iterations, channels, _ptr(out))
+    return out
+
+
+def get_pca_bound(covar10, channels: int) -> float:
+    """get_pca_bound (common.rs:208-245)."""
+    c = _f32(covar10, 10)
+    return float(lib().oracle_get_pca_bound(_ptr(c), channels))
+
+
+def block_pca_axis(block64, mask: int, channels: int):
+    """block_pca_axis (common.rs:370-394): (axis, dc)."""
+    b = _f32(block64, 64)
+    axis = np.zeros(4, np.float32)
+    dc = np.zeros(4, np.float32)
+    lib().oracle_block_pca_axis(_ptr(b), mask, channels, _ptr(axis), _ptr(dc))
+    return axis, dc
+
+
+def block_segment_core(block64, mask: int, channels: int) -> np.ndarray:
+    """block_segment_core (common.rs:483-518): endpoints A (0..3), B (4..7)."""
+    b = _f32(block64, 64)
+    ep = np.zeros(8, np.float32)
+    lib().oracle_block_segment_core(_ptr(b), mask, channels, _ptr(ep))
+    return ep
+
+
+def opt_endpoints(block64, bits: int, qblock2, mask: int, channels: int) -> np.ndarray:
+    """opt_endpoints (common.rs:142-205)."""
+    b = _f32(block64, 64)
+    q = np.ascontiguousarray(qblock2, dtype=np.uint32)
+    ep = np.zeros(8, np.float32)
+    lib().oracle_opt_endpoints(_ptr(b), bits, _ptr(q), mask, channels, _ptr(ep))
+    return ep
+
+
+def block_quant(block64, bits: int, ep8, channels: int):
+    """block_quant (common.rs:422-481) with every texel in subset 0: (err, qblock)."""
+    b = _f32(block64, 64)
+    e = _f32(ep8, 8)
+    q = np.zeros(2, np.uint32)
+    err = lib().oracle_block_quant(_ptr(b), bits, _ptr(e), channels, _ptr(q))
+    return float(err), q
+
+
+def compute_stats_masked(block64, mask: int, channels: int) -> np.ndarray:
+    b = _f32(block64, 64)
+    st = np.zeros(15, np.float32)
+    lib().oracle_compute_stats_masked(_ptr(b), mask, channels, _ptr(st))
+    return st
+
+
+def block_pca_bound_split(block64, mask: int, channels: int) -> float:
+    """block_pca_bound_split (common.rs:396-420) with the block's full stats."""
+    b = _f32(block64, 64)
+    return float(lib().oracle_block_pca_bound_split(_ptr(b), mask, channels))
+
+
+def bc7_ep_quant_pair(mode: int, ep8, channels: int) -> np.ndarray:
+    """ep_quant0367 / ep_quant1 / ep_quant245 of one endpoint pair (bc7.rs:102-189)."""
+    e = _f32(ep8, 8)
+    q = np.zeros(8, np.int32)
+    lib().oracle_bc7_ep_quant_pair(mode, _ptr(e), channels, _ptr(q))
+    return q
+
+
+def bc7_channel_opt_endpoints(channel16, bits: int, qblock2) -> np.ndarray:
+    """channel_opt_endpoints (bc7.rs:322-369)."""
+    c = _f32(channel16, 16)
+    q = np.ascontiguousarray(qblock2, dtype=np.uint32)
+    ep = np.zeros(2, np.float32)
+    lib().oracle_bc7_channel_opt_endpoints(_ptr(c), bits, _ptr(q), _ptr(ep))
+    return ep
+
+
+def bc7_channel_opt_quant(channel16, bits: int, ep2):
+    """channel_opt_quant (bc7.rs:278-320): (err, qblock)."""
+    c = _f32(channel16, 16)
+    e = _f32(ep2, 2)
+    q = np.zeros(2, np.uint32)
+    err = lib().oracle_bc7_channel_opt_quant(_ptr(c), bits, _ptr(e), _ptr(q))
+    return float(err), q
+
+
+def bc1_enc_rgb565(r: float, g: float, b: float) -> int:
+    """enc_rgb565 (bc1_to_5.rs:268-278)."""
+    return int(lib().oracle_bc1_enc_rgb565(r, g, b))
+
+
+def bc1_dec_rgb565(p: int) -> np.ndarray:
+    out = np.zeros(3, np.float32)
+    lib().oracle_bc1_dec_rgb565(p, _ptr(out))
+    return out
+
+
+def bc1_fast_quant(block64, p0: int, p1: int) -> int:
+    """fast_quant (bc1_to_5.rs:280-321): 2-bit indices, texel k at bits 2k."""
+    b = _f32(block64, 64)
+    return int(lib().oracle_bc1_fast_quant(_ptr(b), p0, p1))
+
+
+def bc1_refine(block64, bits: int):
+    """bc1_refine (bc1_to_5.rs:323-374): (colours c0 c1 before 565 rounding, codes)."""
+    b = _f32(block64, 64)
+    col = np.zeros(6, np.float32)
+    pe = np.zeros(2, np.int32)
+    lib().oracle_bc1_refine(_ptr(b), bits, _ptr(col), _ptr(pe))
+    return col.reshape(2, 3), pe
+
+
+def bc1_axis(block64):
+    """BC1's principal axis and mean (bc1_to_5.rs:157-223, 386-397)."""
+    b = _f32(block64, 64)
+    axis = np.zeros(3, np.float32)
+    dc = np.zeros(3, np.float32)
+    lib().oracle_bc1_axis(_ptr(b), _ptr(axis), _ptr(dc))
+    return axis, dc
