@@ -581,9 +581,12 @@ def test_bc7_endpoint_quantisers_are_nearest_lattice_points(mode):
     rng = np.random.default_rng(70 + mode)
     vals = np.concatenate([rng.uniform(-30, 285, 4000), rng.uniform(0, 255, 4000),
                            np.arange(256, dtype=np.float64)]).astype(np.float32)
-    for it in range(1500):
+    ints = np.arange(256, dtype=np.float32)
+    for it in range(1200):
         channels = 3 if mode in (1, 2, 4, 5) else int(rng.integers(3, 5))
-        ep = vals[rng.integers(0, vals.size, 8)]
+        # Every third pair is integer-valued: there the reference's f32 error
+        # sums are exact, so exact p-bit ties occur and must go to b = 1.
+        ep = ints[rng.integers(0, 256, 8)] if it % 3 == 0 else vals[rng.integers(0, vals.size, 8)]
         got = oracle_lib.bc7_ep_quant_pair(mode, ep, channels)
         if mode in (2, 4, 5):
             L = 1 << (7 if mode == 5 else 5)
