@@ -9,7 +9,7 @@ Mirrors `GpuBlockCompressor` (src/block_compressor.rs:45-62):
 
 `texture` is a device tensor (or a DLPack-exporting array) whose dtype and
 channel count give the texture view's format: uint8 RGBA8 / BGRA8 / RG8 / R8
-for BC1-BC5 and BC7, float16 or float32 RGBA for BC6H; `buffer` is a
+for every format, float16 or float32 RGBA for BC6H only; `buffer` is a
 contiguous uint8 torch tensor on the same device.  The argument checks are the reference's asserts
 (block_compressor.rs:352-373), raised as BcnError.  compress() enqueues the
 kernels on `stream` (default: torch's current stream) and empties the queue.
@@ -83,6 +83,7 @@ class GpuBlockCompressor:
             raise TypeError("buffer must be a contiguous uint8 tensor")
         if not texture.is_cuda or not buffer.is_cuda:
             raise ValueError("texture and buffer must be device tensors")
+        check_region(texture, width)
         pitch = texture.stride(0) * texture.element_size()
         settings = variant.settings_bytes()
         sbuf = ctypes.create_string_buffer(settings) if settings else None
@@ -109,13 +110,22 @@ class GpuBlockCompressor:
         return int(_capi.lib().bcn_task_count(self._handle))
 
     def compress(self, stream=None) -> None:
+        """compress (src/block_compressor.rs:540-590): enqueues every queued
+        task on `stream` (a torch stream, a raw hipStream_t handle, or None for
+        torch's current stream) and empties the queue.  Each texture and
+        buffer is marked as used on that stream (record_stream), so torch's
+        caching allocator does not hand their memory to a new tensor before
+        the kernels that read and write it have run."""
+        import torch
+
         if stream is None:
-            handle = _current_stream_handle(self.device)
+            stream = torch.cuda.current_stream(self.device)
         elif isinstance(stream, int):
-            handle = stream
-        else:
-            handle = stream.cuda_stream
-        status = _capi.lib().bcn_compress(self._handle, ctypes.c_void_p(handle))
+            stream = torch.cuda.ExternalStream(stream, device=self.device)
+        status = _capi.lib().bcn_compress(self._handle, ctypes.c_void_p(stream.cuda_stream))
+        if status == _capi.BCN_OK:
+            for t in self._keepalive:
+                t.record_stream(stream)
         self._keepalive.clear()
         _capi.check(status, self._handle)
 
@@ -135,6 +145,17 @@ SOURCE_FORMATS = {
     "rgba16f": _capi.SRC_RGBA16F,
     "rgba32f": _capi.SRC_RGBA32F,
 }
+
+
+def check_region(texture, width: int) -> None:
+    """The region's width fits the texture view.  The C ABI sees a pointer
+    and a row pitch only, so a row-strided view of a larger texture (e.g.
+    big[:, 192:256]) would otherwise be read past its right edge, and on the
+    last rows past the allocation; the reference's textureLoad stays inside
+    the view.  Rows (height, texture_y_offset) are checked by the C ABI
+    against the view's height."""
+    if width > int(texture.shape[1]):
+        raise ValueError(f"width {width} exceeds the texture view's width {int(texture.shape[1])}")
 
 
 def import_texture(obj):
@@ -214,9 +235,18 @@ class BlockCompressionBatch:
         compressor._keepalive.clear()
 
     def launch(self, stream=None, graph: bool = False) -> None:
+        import torch
+
         fn = _capi.lib().bcn_batch_launch_graph if graph else _capi.lib().bcn_batch_launch
-        _capi.check(fn(self._handle, ctypes.c_void_p(_stream_handle(self.device, stream))),
-                    batch=self._handle)
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device)
+        elif isinstance(stream, int):
+            stream = torch.cuda.ExternalStream(stream, device=self.device)
+        _capi.check(fn(self._handle, ctypes.c_void_p(stream.cuda_stream)), batch=self._handle)
+        # As in GpuBlockCompressor.compress: the memory stays reserved for the
+        # launch stream's work even after the batch lets go of the tensors.
+        for t in self._keepalive:
+            t.record_stream(stream)
 
     def autotune(self, reps: int = 2, stream=None) -> dict:
         cfg = _capi.BatchConfig()

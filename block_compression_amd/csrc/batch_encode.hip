@@ -327,8 +327,9 @@ int bcn_batch_create(bcn_compressor* c, const bcn_batch_config* config, bcn_batc
   bcn_batch* b = new (std::nothrow) bcn_batch();
   if (!b) return BCN_ERR_INVALID_ARGUMENT;
   b->device = c->device;
-  std::vector<bcn::Task> tasks;
-  tasks.swap(c->tasks);
+  // The compressor keeps its tasks until the batch exists: on any failure
+  // below they stay queued, as if the call had not been made.
+  const std::vector<bcn::Task>& tasks = c->tasks;
   // Queue tasks, then a stable sort by block cost (most expensive first).
   struct Entry {
     bcn::QueueTask q;
@@ -392,6 +393,7 @@ int bcn_batch_create(bcn_compressor* c, const bcn_batch_config* config, bcn_batc
     bcn_batch_destroy(b);
     return st;
   }
+  c->tasks.clear();
   c->last_error.clear();
   *out = b;
   return BCN_OK;

@@ -2,7 +2,8 @@
 //
 // Replaces compress_bc6h (src/shader/bc6h.wgsl:1401-1424).  Each lane reads
 // its block of RGBA16F texels as four 32-byte rows (RGBA32F: 64-byte rows,
-// rounded to halves) and keeps the 48 scaled channel values in registers;
+// rounded to halves; 8-bit unorm layouts: c/255 rounded to halves) and keeps
+// the 48 scaled channel values in registers;
 // the 32 partition keys live in LDS, one row per lane, interleaved by lane.
 #include "bc6h_block.hpp"
 #include "bcn_internal.hpp"
@@ -44,13 +45,13 @@ __global__ __launch_bounds__(kBc6hGroup) __attribute__((amdgpu_waves_per_eu(SLOW
 }
 
 hipError_t launch_bc6h(const Task& t, hipStream_t stream) {
-  if (t.format != BCN_SRC_RGBA16F && t.format != BCN_SRC_RGBA32F) return hipErrorNotSupported;
+  if (source_texel_bytes(t.format) == 0) return hipErrorNotSupported;
   Bc6hParams s;
   static_assert(sizeof(Bc6hParams) == 20, "BC6HSettings is 5 x u32");
   __builtin_memcpy(&s, t.settings, sizeof(s));
   uint32_t bw = blocks_x(t), nblocks = bw * blocks_y(t);
   if (nblocks == 0) return hipSuccess;
-  const uintptr_t align = t.format == BCN_SRC_RGBA32F ? 15u : 7u;
+  const uintptr_t align = t.format == BCN_SRC_RGBA32F ? 15u : (t.format == BCN_SRC_RGBA16F ? 7u : 3u);
   if ((((uintptr_t)t.texture | (uintptr_t)t.pitch) & align) != 0) return hipErrorInvalidValue;
   dim3 grid((nblocks + kBc6hGroup - 1) / kBc6hGroup), block(kBc6hGroup);
   uint32_t* dst = reinterpret_cast<uint32_t*>(t.blocks);

@@ -93,11 +93,10 @@ int bcn_add_compression_task(bcn_compressor* c, bcn_variant variant, const void*
   const bool float_src = texture_format == BCN_SRC_RGBA16F || texture_format == BCN_SRC_RGBA32F;
   if (float_src && !is_bc6h(variant))
     return fail(c, BCN_ERR_UNSUPPORTED, "a float source is only accepted for BC6H");
-  // The reference's RGBA8 route into BC6H (compress_rgba8, encode.rs:130-137)
-  // converts through powf-based sRGB decoding; the GPU path takes half-float
-  // (or float) texels only (compress_rgba16, encode.rs:211-251).
-  if (!float_src && is_bc6h(variant))
-    return fail(c, BCN_ERR_UNSUPPORTED, "BC6H takes an RGBA16F or RGBA32F source");
+  // BC6H from an 8-bit layout reads it as the replaced GPU path reads a unorm
+  // view (block_compressor.rs:312, bc6h.wgsl:64-85): c/255 rounded to a half.
+  // (The reference's CPU compress_rgba8 route into BC6H decodes sRGB with
+  // powf instead, encode.rs:130-137, bc6h.rs:20-27; it is not this path.)
   bcn::Task t{};
   t.variant = variant;
   if (variant == BCN_BC7) {

@@ -95,11 +95,34 @@ __device__ __forceinline__ uint16_t f32_to_f16_bits(float f) {
   return __builtin_bit_cast(uint16_t, (_Float16)f);
 }
 
-// BC6H source block: the R, G, B half bit patterns, bits[c*16 + k].
+// A half texel as textureLoad + f32_to_f16_bits hands it to the BC6H search:
+// -0 reads as +0 (bc6h.wgsl:61-63), every other pattern as itself.
+__device__ __forceinline__ uint16_t half_texel(uint32_t h) {
+  return h == 0x8000u ? (uint16_t)0 : (uint16_t)h;
+}
+
+// An 8-bit unorm channel as the BC6H search sees it through a unorm view:
+// textureLoad's c / 255 in f32 (correctly rounded), then f32_to_f16_bits
+// (round to nearest even; every such value is 0 or a normal half).
+__device__ __forceinline__ uint16_t unorm8_texel(uint32_t c) {
+  return f32_to_f16_bits((float)c / 255.0f);
+}
+
+// BC6H source block: the R, G, B half bit patterns, bits[c*16 + k], from an
+// RGBA16F, RGBA32F or 8-bit unorm (RGBA8, BGRA8, R8, RG8) layout.
 __device__ __forceinline__ void load_block_rgb16f(const uint8_t* src, uint32_t pitch, uint32_t xx,
                                                   uint32_t yy, uint32_t fmt, uint16_t bits[48]) {
   const uint8_t* base = src + (size_t)yy * 4 * pitch;
-  if (fmt == BCN_SRC_RGBA32F) {
+  if (fmt != BCN_SRC_RGBA32F && fmt != BCN_SRC_RGBA16F) {
+    uint32_t px[16];
+    load_block_rgba8(src, pitch, xx, yy, fmt, false, px);
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      bits[k] = unorm8_texel(px[k] & 0xFFu);
+      bits[16 + k] = unorm8_texel((px[k] >> 8) & 0xFFu);
+      bits[32 + k] = unorm8_texel((px[k] >> 16) & 0xFFu);
+    }
+  } else if (fmt == BCN_SRC_RGBA32F) {
     const uint8_t* p = base + (size_t)xx * 64;
 #pragma unroll
     for (int y = 0; y < 4; y++) {
@@ -122,9 +145,9 @@ __device__ __forceinline__ void load_block_rgb16f(const uint8_t* src, uint32_t p
       for (int x = 0; x < 4; x++) {
         uint2 t = row[x];  // R | G << 16, B | A << 16
         int k = y * 4 + x;
-        bits[k] = (uint16_t)(t.x & 0xFFFFu);
-        bits[16 + k] = (uint16_t)(t.x >> 16);
-        bits[32 + k] = (uint16_t)(t.y & 0xFFFFu);
+        bits[k] = half_texel(t.x & 0xFFFFu);
+        bits[16 + k] = half_texel(t.x >> 16);
+        bits[32 + k] = half_texel(t.y & 0xFFFFu);
       }
     }
   }

@@ -139,6 +139,12 @@ def refine_bc7(blocks, target, width: int, height: int, steps: int = 64, lr: flo
         raise ValueError("target texels must be packed")
     if out is None:
         out = torch.empty_like(blocks)
+    elif _check_blocks(out, BC7) != n or out.device != blocks.device:
+        raise ValueError("out must hold as many blocks as `blocks`, on the same device")
+    if (blocks.data_ptr() | out.data_ptr() | target.data_ptr()) & 15:
+        raise ValueError("blocks, out and target must be 16-byte aligned")
+    if target.device != blocks.device:
+        raise ValueError("target must be on the blocks' device")
     s0 = torch.empty(n, dtype=torch.float32, device=blocks.device)
     s1 = torch.empty(n, dtype=torch.float32, device=blocks.device)
     wsize = int(_capi.lib().bcn_bc7_refine_workspace_size(n))

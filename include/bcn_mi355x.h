@@ -45,15 +45,18 @@ typedef enum bcn_variant {
  * textureLoad (src/block_compressor.rs:300-313).  The encoders work on the
  * raw texel values: an sRGB texture is passed as its bytes (the reference's
  * non-sRGB view of it).  A channel a layout lacks reads as textureLoad gives
- * it: G = B = 0, A = 255.  BC1-BC5 and BC7 take the 8-bit layouts; BC6H
- * takes RGBA16F or RGBA32F. */
+ * it: G = B = 0, A = 255.  BC1-BC5 and BC7 take the 8-bit layouts.  BC6H
+ * takes every layout, as the replaced shader reads them (bc6h.wgsl:59-85):
+ * RGBA16F half bits as they are except -0, which reads as +0; RGBA32F and
+ * the 8-bit unorm layouts (c/255 in f32) rounded to the nearest half, ties
+ * to even, every zero as +0. */
 typedef enum bcn_source_format {
   BCN_SRC_RGBA8 = 0,    /* Rgba8Unorm: 4 x u8 per texel, read as exact integers */
-  BCN_SRC_RGBA16F = 1,  /* Rgba16Float: 4 x IEEE binary16 per texel (BC6H) */
+  BCN_SRC_RGBA16F = 1,  /* Rgba16Float: 4 x IEEE binary16 per texel (BC6H only) */
   BCN_SRC_R8 = 2,       /* R8Unorm: 1 byte per texel -> (r, 0, 0, 255) */
   BCN_SRC_RG8 = 3,      /* Rg8Unorm: 2 bytes per texel -> (r, g, 0, 255) */
   BCN_SRC_BGRA8 = 4,    /* Bgra8Unorm: 4 bytes per texel, B first */
-  BCN_SRC_RGBA32F = 5   /* Rgba32Float: 4 x f32 per texel (BC6H); each channel is
+  BCN_SRC_RGBA32F = 5   /* Rgba32Float: 4 x f32 per texel (BC6H only); each channel is
                            rounded to the nearest half (ties to even) and every
                            zero reads as +0, as bc6h.wgsl:59-70 converts it */
 } bcn_source_format;
@@ -244,10 +247,12 @@ void bcn_partition_tables(uint32_t* pattern, uint32_t* mask01, uint8_t* anchor1,
  * No counterpart in the reference, which records one compute dispatch per
  * queued task (GpuBlockCompressor::compress, src/block_compressor.rs:540-590).
  * A batch takes every task queued on a compressor -- any number of textures
- * and mip levels, mixed formats -- and encodes the BC1-BC5 and BC7 ones in ONE
- * persistent kernel over a device-resident block work queue (chunks of blocks
- * of one task, most expensive first); BC6H tasks of the batch are launched
- * per task after it.  The output bytes are the ones bcn_compress writes. */
+ * and mip levels, mixed formats -- and encodes the BC1-BC5 and BC7 ones with
+ * persistent kernels over a device-resident block work queue (chunks of
+ * blocks of one task, most expensive first), one kernel per kernel class
+ * (BC7 with alpha, BC7 opaque, BC1-BC5) so that each class keeps its own
+ * register allocation; BC6H tasks of the batch are launched per task after
+ * them.  The output bytes are the ones bcn_compress writes. */
 typedef struct bcn_batch bcn_batch;
 
 typedef struct bcn_batch_config {
@@ -268,7 +273,8 @@ typedef struct bcn_batch_info {
 
 /* Moves every task queued on `c` into a new batch and uploads its work queue
  * (the only allocation and host-to-device copy).  config NULL = defaults.
- * The textures and buffers the tasks name must outlive the batch. */
+ * On failure the tasks stay queued on `c`.  The textures and buffers the
+ * tasks name must outlive the batch. */
 int bcn_batch_create(bcn_compressor* c, const bcn_batch_config* config, bcn_batch** out);
 
 /* Enqueues the batch on hip_stream: a one-thread kernel that rewinds the

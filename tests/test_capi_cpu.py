@@ -58,6 +58,8 @@ def test_status_strings():
     lib = _capi.lib()
     for s in range(9):
         assert lib.bcn_status_string(s)
+        assert lib.bcn_status_string(s) != b"unknown status"
+    assert lib.bcn_status_string(9) == b"unknown status"
 
 
 def test_add_task_preconditions_without_gpu():
@@ -67,3 +69,4 @@ def test_add_task_preconditions_without_gpu():
     rc = lib.bcn_add_compression_task(None, 6, None, 0, None, 0, 16, 4, 4, 4, None, 16, 0, 0)
     assert rc == 1
     assert lib.bcn_compress(None, None) == 1
+

@@ -177,3 +177,23 @@ def test_mip_chain_workload_batched_vs_plain_and_oracle():
         want = oracle_lib.compress_rgba8(v.tag, v.settings_bytes(), lv.texels.cpu().numpy(), threads=8)
         assert np.array_equal(b[sl], want), (lv.texture, lv.level, lv.fmt)
     batch.close()
+
+
+def test_failed_batch_creation_keeps_the_tasks_queued():
+    """A batch that cannot be created (here: a chunk size that is not a
+    multiple of 64) leaves every task queued on the compressor, which can
+    then still compress them."""
+    from block_compression_amd import BcnError, GpuBlockCompressor
+
+    cases = CASES[:3]
+    texs = _setup(cases)
+    bufs = _bufs(cases)
+    c = GpuBlockCompressor()
+    _queue(c, cases, bufs, texs)
+    with pytest.raises(BcnError):
+        c.create_batch({"chunk_blocks_bc7": 65, "chunk_blocks_bc15": 1024, "waves_per_cu": 0})
+    assert c.task_count() == len(cases)
+    c.compress()
+    torch.cuda.synchronize()
+    for got, want in zip([b.cpu().numpy() for b in bufs], _per_task(cases, texs)):
+        assert np.array_equal(got, want)

@@ -50,17 +50,6 @@ def test_bc6h_slow_2048_sampled():
     assert np.array_equal(got[ids], np.stack(ref))
 
 
-def test_bc6h_rejects_rgba8_source():
-    from block_compression_amd import BcnError, GpuBlockCompressor
-
-    tex = torch.zeros((4, 4, 4), dtype=torch.uint8, device="cuda")
-    buf = torch.zeros(16, dtype=torch.uint8, device="cuda")
-    c = GpuBlockCompressor()
-    with pytest.raises(BcnError, match="UNSUPPORTED"):
-        c.add_compression_task(CompressionVariant.BC6H(BC6HSettings.basic()), tex, 4, 4, buf)
-        c.compress()
-
-
 @pytest.mark.parametrize("tag,name", [(0, "bc1"), (1, "bc2"), (2, "bc3"), (3, "bc4"), (4, "bc5"), (6, "bc7")])
 def test_gpu_decode_matches_oracle_random_blocks(tag, name):
     from block_compression_amd import decode_rgba8

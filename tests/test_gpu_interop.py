@@ -171,13 +171,16 @@ def test_layout_variant_checks():
     buf = torch.zeros(4096, dtype=torch.uint8, device="cuda")
     with pytest.raises(BcnError, match="UNSUPPORTED"):
         c.add_compression_task(_variant(6), torch.zeros((8, 8, 4), dtype=torch.float32, device="cuda"), 8, 8, buf)
-    with pytest.raises(BcnError, match="UNSUPPORTED"):
-        c.add_compression_task(CompressionVariant.BC6H(BC6HSettings.basic()),
-                               torch.zeros((8, 8), dtype=torch.uint8, device="cuda"), 8, 8, buf)
     # RGBA32F rows must keep 16-byte texels aligned: a view starting one
     # float into the allocation is refused.
     raw = torch.zeros(8 * 8 * 4 + 1, dtype=torch.float32, device="cuda")
     with pytest.raises(BcnError, match="MISALIGNED"):
         c.add_compression_task(CompressionVariant.BC6H(BC6HSettings.basic()),
                                raw[1:].view(8, 8, 4), 8, 8, buf)
+    assert c.task_count() == 0
+    # A view narrower than the region is refused before any launch (the C ABI
+    # sees only a pointer and a pitch).
+    big = torch.zeros((16, 64, 4), dtype=torch.uint8, device="cuda")
+    with pytest.raises(ValueError, match="width"):
+        c.add_compression_task(_variant(6), big[:, 16:32], 32, 16, buf)
     assert c.task_count() == 0
