@@ -27,7 +27,9 @@ STATUS_NAMES = {
     6: "BCN_ERR_MISALIGNED",
     7: "BCN_ERR_HIP",
     8: "BCN_ERR_NO_DEVICE",
+    9: "BCN_ERR_TOO_LARGE",
 }
+NOALLOC_MAX_BLOCKS = 1 << 20
 
 SRC_RGBA8 = 0
 SRC_RGBA16F = 1
@@ -69,6 +71,7 @@ SIGNATURES = [
     ("bcn_decode_with_endpoints_backward", _i32, [_i32, _vp, _vp, _u32, _vp, _vp]),
     ("bcn_bc7_refine", _i32, [_vp, _vp, _u32, _u32, _vp, _u32, _vp, _vp, _vp, _vp, _sz, _vp]),
     ("bcn_bc7_refine_workspace_size", _sz, [_u32]),
+    ("bcn_bc7_refine_noalloc", _i32, [_vp, _vp, _u32, _u32, _vp, _u32, _vp, _vp, _vp, _vp]),
 ]
 
 

@@ -48,6 +48,7 @@ const char* bcn_status_string(int status) {
     case BCN_ERR_MISALIGNED: return "misaligned pointer or pitch";
     case BCN_ERR_HIP: return "HIP error";
     case BCN_ERR_NO_DEVICE: return "no such HIP device";
+    case BCN_ERR_TOO_LARGE: return "input larger than the allocation-free entry point's scratch";
     default: return "unknown status";
   }
 }

@@ -1,6 +1,8 @@
 // Decode adjoints (BC7, BC1) and gradient endpoint refinement (BC7): one lane
 // per block.  The maths is in decode_adjoint.hpp; these are the kernels and
 // the C ABI.
+#include <mutex>
+
 #include "bcn_internal.hpp"
 #include "decode_adjoint.hpp"
 
@@ -484,6 +486,84 @@ int check_launch() { return hipGetLastError() == hipSuccess ? BCN_OK : BCN_ERR_H
 
 bool fmt_ok(int fmt) { return fmt == BCN_BC7 || fmt == BCN_BC1; }
 
+// Argument checks shared by bcn_bc7_refine and bcn_bc7_refine_noalloc; no
+// device work.
+int refine_check(const void* blocks_in, const void* blocks_out, uint32_t width, uint32_t height,
+                 const float* target, uint32_t target_pitch, const bcn_refine_params* params,
+                 const float* sse_before, const float* sse_after) {
+  if (!params || width % 4 || height % 4) return BCN_ERR_INVALID_ARGUMENT;
+  if (params->channels != 3 && params->channels != 4) return BCN_ERR_INVALID_ARGUMENT;
+  if (width == 0 || height == 0) return BCN_OK;
+  if (!blocks_in || !blocks_out || !target) return BCN_ERR_INVALID_ARGUMENT;
+  if ((uint64_t)target_pitch < (uint64_t)width * 16) return BCN_ERR_INVALID_ARGUMENT;
+  if ((((uintptr_t)blocks_in | (uintptr_t)blocks_out | (uintptr_t)target | target_pitch) & 15u) ||
+      ((uintptr_t)sse_before & 3u) || ((uintptr_t)sse_after & 3u))
+    return BCN_ERR_MISALIGNED;
+  return BCN_OK;
+}
+
+// The refinement's launches on one workspace: clear the header, count blocks
+// per mode, offsets, scatter (and refine the blocks no sorted wave takes),
+// then the sorted, mode-specialised refinement.  Every word the later
+// kernels read is written by an earlier one in this call.
+int refine_launch(const void* blocks_in, void* blocks_out, uint32_t width, uint32_t height,
+                  const float* target, uint32_t target_pitch, const bcn_refine_params& params,
+                  float* sse_before, float* sse_after, uint32_t* ws, hipStream_t s) {
+  const uint32_t bw = width / 4, nblocks = bw * (height / 4);
+  bcn::RefineParams rp;
+  rp.steps = params.steps;
+  rp.lr = params.lr;
+  rp.beta1 = params.beta1;
+  rp.beta2 = params.beta2;
+  rp.eps = params.eps;
+  rp.channels = params.channels;
+  rp.reassign = params.reassign;
+  rp.pbits = params.pbits;
+  const uint32_t* in = static_cast<const uint32_t*>(blocks_in);
+  uint32_t* out = static_cast<uint32_t*>(blocks_out);
+  const uint8_t* tg = reinterpret_cast<const uint8_t*>(target);
+  const uint32_t waves = (nblocks + bcn::kGradGroup - 1) / bcn::kGradGroup;
+  hipLaunchKernelGGL(bcn::refine_clear_kernel, dim3(1), dim3(64), 0, s, ws);
+  hipLaunchKernelGGL(bcn::refine_count_kernel, dim3((nblocks + 255) / 256), dim3(256), 0, s, in, nblocks, ws);
+  hipLaunchKernelGGL(bcn::refine_offsets_kernel, dim3(1), dim3(64), 0, s, ws);
+  hipLaunchKernelGGL(bcn::refine_scatter_kernel, dim3(waves), dim3(bcn::kGradGroup), 0, s, in, out, bw, nblocks,
+                     tg, target_pitch, rp, sse_before, sse_after, ws);
+  // Sixteen blocks per wave, at most one partial wave per mode.
+  const uint32_t qwaves = (nblocks + bcn::kQuadBlocks - 1) / bcn::kQuadBlocks + 8;
+  hipLaunchKernelGGL(bcn::refine_sorted_kernel, dim3(qwaves), dim3(bcn::kGradGroup), 0, s, in, out, bw, tg,
+                     target_pitch, rp, sse_before, sse_after, ws);
+  return check_launch();
+}
+
+// ---- the allocation-free refinement's scratch --------------------------------
+//
+// A fixed device array replaces the caller's workspace: header plus one word
+// per block for up to kNoAllocMaxBlocks blocks (4096 x 4096 texels), 4 MiB
+// per GPU, held until the process ends.  In host code `scratch` names a
+// host-side stand-in, not device memory: only the address that
+// hipGetSymbolAddress returns for the current device is passed to kernels.
+constexpr uint32_t kNoAllocMaxBlocks = BCN_NOALLOC_MAX_BLOCKS;
+__device__ __align__(256) uint32_t scratch[bcn::kRefineWsHeader + kNoAllocMaxBlocks];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[kMaxDevices];
+
+// The current device's copy of `scratch`, looked up once per device.
+int noalloc_scratch(uint32_t** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0) return BCN_ERR_NO_DEVICE;
+  if (dev >= kMaxDevices) return BCN_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return BCN_ERR_HIP;
+    g_scratch_ptr[dev] = static_cast<uint32_t*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return BCN_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -540,41 +620,31 @@ int bcn_bc7_refine(const void* blocks_in, void* blocks_out, uint32_t width, uint
                    const float* target, uint32_t target_pitch, const bcn_refine_params* params,
                    float* sse_before, float* sse_after, void* workspace, size_t workspace_size,
                    void* hip_stream) {
-  if (!params || width % 4 || height % 4) return BCN_ERR_INVALID_ARGUMENT;
-  if (params->channels != 3 && params->channels != 4) return BCN_ERR_INVALID_ARGUMENT;
-  const uint32_t bw = width / 4, nblocks = bw * (height / 4);
-  if (nblocks == 0) return BCN_OK;
-  if (!blocks_in || !blocks_out || !target || !workspace) return BCN_ERR_INVALID_ARGUMENT;
+  int rc = refine_check(blocks_in, blocks_out, width, height, target, target_pitch, params, sse_before,
+                        sse_after);
+  if (rc != BCN_OK || width == 0 || height == 0) return rc;
+  const uint32_t nblocks = (width / 4) * (height / 4);
+  if (!workspace) return BCN_ERR_INVALID_ARGUMENT;
   if (workspace_size < bcn_bc7_refine_workspace_size(nblocks)) return BCN_ERR_BUFFER_TOO_SMALL;
-  if ((uint64_t)target_pitch < (uint64_t)width * 16) return BCN_ERR_INVALID_ARGUMENT;
-  if ((((uintptr_t)blocks_in | (uintptr_t)blocks_out | (uintptr_t)target | target_pitch) & 15u) ||
-      ((uintptr_t)sse_before & 3u) || ((uintptr_t)sse_after & 3u) || ((uintptr_t)workspace & 3u))
-    return BCN_ERR_MISALIGNED;
-  bcn::RefineParams rp;
-  rp.steps = params->steps;
-  rp.lr = params->lr;
-  rp.beta1 = params->beta1;
-  rp.beta2 = params->beta2;
-  rp.eps = params->eps;
-  rp.channels = params->channels;
-  rp.reassign = params->reassign;
-  rp.pbits = params->pbits;
-  hipStream_t s = static_cast<hipStream_t>(hip_stream);
-  const uint32_t* in = static_cast<const uint32_t*>(blocks_in);
-  uint32_t* out = static_cast<uint32_t*>(blocks_out);
-  uint32_t* ws = static_cast<uint32_t*>(workspace);
-  const uint8_t* tg = reinterpret_cast<const uint8_t*>(target);
-  const uint32_t waves = (nblocks + bcn::kGradGroup - 1) / bcn::kGradGroup;
-  hipLaunchKernelGGL(bcn::refine_clear_kernel, dim3(1), dim3(64), 0, s, ws);
-  hipLaunchKernelGGL(bcn::refine_count_kernel, dim3((nblocks + 255) / 256), dim3(256), 0, s, in, nblocks, ws);
-  hipLaunchKernelGGL(bcn::refine_offsets_kernel, dim3(1), dim3(64), 0, s, ws);
-  hipLaunchKernelGGL(bcn::refine_scatter_kernel, dim3(waves), dim3(bcn::kGradGroup), 0, s, in, out, bw, nblocks,
-                     tg, target_pitch, rp, sse_before, sse_after, ws);
-  // Sixteen blocks per wave, at most one partial wave per mode.
-  const uint32_t qwaves = (nblocks + bcn::kQuadBlocks - 1) / bcn::kQuadBlocks + 8;
-  hipLaunchKernelGGL(bcn::refine_sorted_kernel, dim3(qwaves), dim3(bcn::kGradGroup), 0, s, in, out, bw, tg,
-                     target_pitch, rp, sse_before, sse_after, ws);
-  return check_launch();
+  if ((uintptr_t)workspace & 3u) return BCN_ERR_MISALIGNED;
+  return refine_launch(blocks_in, blocks_out, width, height, target, target_pitch, *params, sse_before,
+                       sse_after, static_cast<uint32_t*>(workspace), static_cast<hipStream_t>(hip_stream));
+}
+
+int bcn_bc7_refine_noalloc(const void* blocks_in, void* blocks_out, uint32_t width, uint32_t height,
+                           const float* target, uint32_t target_pitch, const bcn_refine_params* params,
+                           float* sse_before, float* sse_after, void* hip_stream) {
+  if (params && width % 4 == 0 && height % 4 == 0 &&
+      (uint64_t)(width / 4) * (height / 4) > kNoAllocMaxBlocks)
+    return BCN_ERR_TOO_LARGE;
+  int rc = refine_check(blocks_in, blocks_out, width, height, target, target_pitch, params, sse_before,
+                        sse_after);
+  if (rc != BCN_OK || width == 0 || height == 0) return rc;
+  uint32_t* ws = nullptr;
+  rc = noalloc_scratch(&ws);
+  if (rc != BCN_OK) return rc;
+  return refine_launch(blocks_in, blocks_out, width, height, target, target_pitch, *params, sse_before,
+                       sse_after, ws, static_cast<hipStream_t>(hip_stream));
 }
 
 }  // extern "C"

@@ -70,7 +70,8 @@ typedef enum bcn_status {
   BCN_ERR_UNSUPPORTED = 5,          /* e.g. RGBA16F source for a non-BC6H variant */
   BCN_ERR_MISALIGNED = 6,           /* source/destination not 4-byte aligned */
   BCN_ERR_HIP = 7,                  /* a HIP runtime call failed; see bcn_last_error */
-  BCN_ERR_NO_DEVICE = 8
+  BCN_ERR_NO_DEVICE = 8,
+  BCN_ERR_TOO_LARGE = 9             /* input beyond the fixed scratch of an allocation-free entry point */
 } bcn_status;
 
 /* BC6HSettings, 5 x u32 repr(C) (src/settings.rs:9-15). */
@@ -357,6 +358,23 @@ int bcn_bc7_refine(const void* blocks_in, void* blocks_out, uint32_t width, uint
 /* Bytes of device workspace bcn_bc7_refine needs for nblocks blocks (a sort
  * of the blocks by mode, so that each wave refines blocks of one mode). */
 size_t bcn_bc7_refine_workspace_size(uint32_t nblocks);
+
+/* Largest block count bcn_bc7_refine_noalloc accepts: 4096 x 4096 texels. */
+#define BCN_NOALLOC_MAX_BLOCKS 1048576u
+
+/* bcn_bc7_refine without a workspace argument and without any allocation.
+ * No counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The workspace is a fixed scratch array in the library's
+ * device code, one per GPU, of 4 MiB + 160 B, held until the process ends;
+ * images of more than BCN_NOALLOC_MAX_BLOCKS blocks are rejected with
+ * BCN_ERR_TOO_LARGE before any device work.  All calls on one GPU share that
+ * array, so calls that overlap on one GPU (on different streams, or from
+ * different threads) race and must be serialised by the caller.  Results are
+ * those of bcn_bc7_refine with the same arguments. */
+int bcn_bc7_refine_noalloc(const void* blocks_in, void* blocks_out, uint32_t width, uint32_t height,
+                           const float* target, uint32_t target_pitch, const bcn_refine_params* params,
+                           float* sse_before, float* sse_after, void* hip_stream);
 
 #ifdef __cplusplus
 }

@@ -56,10 +56,10 @@ def test_blocks_byte_size_matches_lib_rs():
 
 def test_status_strings():
     lib = _capi.lib()
-    for s in range(9):
+    for s in range(10):
         assert lib.bcn_status_string(s)
         assert lib.bcn_status_string(s) != b"unknown status"
-    assert lib.bcn_status_string(9) == b"unknown status"
+    assert lib.bcn_status_string(10) == b"unknown status"
 
 
 def test_add_task_preconditions_without_gpu():
@@ -70,3 +70,16 @@ def test_add_task_preconditions_without_gpu():
     assert rc == 1
     assert lib.bcn_compress(None, None) == 1
 
+
+def test_noalloc_refine_size_and_argument_checks_without_gpu():
+    """bcn_bc7_refine_noalloc rejects an image beyond its fixed scratch
+    (BCN_ERR_TOO_LARGE) and bad arguments before any device work."""
+    lib = _capi.lib()
+    p = _capi.RefineParams(1, 0.5, 0.9, 0.999, 1e-8, 4, 0, 0)
+    big = 4 * int(np.sqrt(_capi.NOALLOC_MAX_BLOCKS))
+    assert lib.bcn_bc7_refine_noalloc(None, None, big, big + 4, None, big * 16, ctypes.byref(p),
+                                      None, None, None) == 9
+    assert lib.bcn_bc7_refine_noalloc(None, None, big, big, None, big * 16, ctypes.byref(p),
+                                      None, None, None) == 1
+    assert lib.bcn_bc7_refine_noalloc(None, None, 6, 4, None, 96, ctypes.byref(p), None, None, None) == 1
+    assert lib.bcn_bc7_refine_noalloc(None, None, 8, 8, None, 128, None, None, None, None) == 1
