@@ -9,9 +9,10 @@ step is also bracketed by HIP events on the launch stream, which gives the
 kernel's average launch duration for the roofline.
 
 Besides the JSON line's throughput, the run checks its own output against the
-CPU oracle (oracle/, a C++ restatement of the reference's CPU encoder) on a
-seeded sample of blocks -- byte identity and the reference harness's PSNR
-(tests/metrics.rs:40-114) -- and times that oracle run as the CPU baseline.
+CPU oracle (oracle/, a C++ restatement of the reference's CPU encoder) on
+every block -- byte identity and the reference harness's per-image PSNR
+(tests/metrics.rs:40-114) -- and times the oracle on the host's CPUs and on
+one core as the CPU baseline.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -39,22 +40,72 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def srgb_to_linear_table():
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads(requested: int):
+    """Oracle threads and a description of the host: every CPU this process
+    may run on (its affinity set), at most 16 -- a GPU box gives one GPU's
+    job a share of 16 CPUs whatever nproc shows."""
+    visible = os.cpu_count() or 1
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = visible
+    threads = requested or max(1, min(16, allowed))
+    return threads, {"cpus_visible": visible, "cpus_allowed": allowed, "cpu_model": cpu_model()}
+
+
+def cpu_baseline_leg(img, settings, nblocks: int, threads: int, host: dict) -> dict:
+    """The oracle (C++ restatement of the reference's CPU encoder, OpenMP over
+    blocks as the crate's rayon feature spreads it) on the same image, timed
+    as BASELINE.md plans it: 1 warm-up, then the median of 5 runs, on all the
+    host CPUs this job may use and on one core.  Bounded seeded samples of
+    the workload's blocks keep the leg to about 10 s."""
     import numpy as np
 
-    v = np.arange(256, dtype=np.float64) / 255.0
-    return np.where(v <= 0.04045, v / 12.92, ((v + 0.055) / 1.055) ** 2.4)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
 
+    def median_rate(ids, nthreads):
+        oracle_lib.compress_rgba8_sample(6, settings.to_bytes(), img, ids[: min(ids.size, 4096)], nthreads)
+        times = []
+        for _ in range(5):
+            t = time.perf_counter()
+            oracle_lib.compress_rgba8_sample(6, settings.to_bytes(), img, ids, nthreads)
+            times.append(time.perf_counter() - t)
+        t = float(np.median(times))
+        return ids.size * 16 / t / 1e6, t, [round(x, 4) for x in times]
 
-def psnr_rgb(orig, dec) -> float:
-    """calculate_image_metrics_rgba8 (tests/metrics.rs:40-114), 3 channels:
-    per-channel MSE in linear light, overall = mean, 0.0 when the MSE is 0."""
-    import numpy as np
-
-    lut = srgb_to_linear_table()
-    d = lut[orig[..., :3]] - lut[dec[..., :3]]
-    mse = (d * d).reshape(-1, 3).mean(axis=0).mean()
-    return 0.0 if mse == 0.0 else 20.0 * math.log10(1.0 / math.sqrt(mse))
+    rng = np.random.default_rng(7)
+    many = np.sort(rng.choice(nblocks, 65536, replace=False))
+    one = np.sort(rng.choice(nblocks, 4096, replace=False))
+    log(f"bench: CPU baseline, {threads} threads (65,536 blocks) and 1 thread (4,096 blocks), median of 5")
+    v_all, t_all, runs_all = median_rate(many, threads)
+    v_one, t_one, runs_one = median_rate(one, 1)
+    return {
+        "value": round(v_all, 3),
+        "unit": "Mpix/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"65,536 seeded random blocks (1,048,576 px) of the same 4096x4096 opaque_basic "
+                  f"workload, C++ oracle, OpenMP over blocks on {threads} threads; 1 warm-up, median of 5 "
+                  f"({t_all:.3f} s)",
+        "runs_s": runs_all,
+        "one_core": {"value": round(v_one, 3), "unit": "Mpix/s", "cores": 1,
+                     "sample": f"4,096 seeded random blocks (65,536 px) of the same workload; 1 warm-up, "
+                               f"median of 5 ({t_one:.3f} s)", "runs_s": runs_one},
+        "host": host,
+        "build": "g++ -O3 -ffp-contract=off -fopenmp, baseline x86-64 (no -march)",
+    }
 
 
 def bench_formats(img, reps: int = 5, launches: int = 20) -> dict:
@@ -196,9 +247,13 @@ def bench_batched(n_textures: int, reps: int, oracle_levels: int, threads: int) 
     stream = torch.cuda.current_stream()
     comp = GpuBlockCompressor(0)
 
-    def timed(fn, n):
+    def timed(fn, n, prep=None):
+        """Best of n: HIP events on the launch stream around fn(); prep()
+        (host-side queuing) runs before the first event, outside the timing."""
         best = None
         for _ in range(n):
+            if prep is not None:
+                prep()
             a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             torch.cuda.synchronize()
             a.record(stream)
@@ -209,22 +264,47 @@ def bench_batched(n_textures: int, reps: int, oracle_levels: int, threads: int) 
             best = ms if best is None else min(best, ms)
         return best
 
+    def wall(fn, n):
+        """Best of n wall-clock seconds of fn() to a device synchronisation."""
+        best = None
+        for _ in range(n):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t) * 1e3
+            best = ms if best is None else min(best, ms)
+        return best
+
     def plain_step():
         workloads.queue(comp, levels, plain)
         comp.compress()
 
     plain_step()
     torch.cuda.synchronize()
-    plain_ms = timed(plain_step, reps)
-    log(f"bench: per-task launches {plain_ms:.2f} ms")
+    # Kernels only: the 3,513 tasks are queued before the first event.
+    plain_ms = timed(comp.compress, reps, prep=lambda: workloads.queue(comp, levels, plain))
+    # Host set-up included: queuing every task, then compress().
+    plain_setup_ms = wall(plain_step, reps)
+    log(f"bench: per-task launches {plain_ms:.2f} ms (kernels), {plain_setup_ms:.2f} ms with queuing")
     workloads.queue(comp, levels, batched)
     batch = comp.create_batch()
     cfg = batch.autotune(reps=1)
     batch.launch(graph=True)
     torch.cuda.synchronize()
     batched_ms = timed(lambda: batch.launch(graph=True), reps)
+
+    def batched_setup_step():
+        workloads.queue(comp, levels, batched)
+        b2 = comp.create_batch(cfg)
+        b2.launch()
+        torch.cuda.synchronize()
+        b2.close()
+
+    batched_setup_ms = wall(batched_setup_step, reps)
     info = batch.info()
-    log(f"bench: batched persistent launch {batched_ms:.2f} ms, config {cfg}")
+    log(f"bench: batched persistent launch {batched_ms:.2f} ms (graph replay), {batched_setup_ms:.2f} ms "
+        f"with queuing and batch creation, config {cfg}")
     same = bool(torch.equal(plain, batched))
     # Oracle on a seeded sample of levels (every level of the sample, all blocks).
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -260,14 +340,20 @@ def bench_batched(n_textures: int, reps: int, oracle_levels: int, threads: int) 
         "batched_ms": round(batched_ms, 3),
         "batched_mpix_s": round(texels / (batched_ms * 1e-3) / 1e6, 1),
         "speedup_vs_plain": round(plain_ms / batched_ms, 3),
+        "plain_with_setup_ms": round(plain_setup_ms, 3),
+        "batched_with_setup_ms": round(batched_setup_ms, 3),
+        "speedup_with_setup": round(plain_setup_ms / batched_setup_ms, 3),
         "autotuned_config": cfg,
         "grid": info["grid"],
         "chunks": info["chunks"],
         "identical_to_plain": same,
         "oracle_blocks_checked": checked,
         "oracle_identical_frac": round(identical / checked, 6) if checked else None,
-        "timing": f"best of {reps} launches each, HIP events on one stream; plain = one kernel per "
-                  "mip level (bcn_compress), batched = bcn_batch_launch_graph",
+        "timing": f"best of {reps} each.  Kernels only (HIP events on the launch stream): plain = "
+                  "compress() of the already-queued tasks, one kernel per mip level as the reference "
+                  "dispatches; batched = one replay of the batch's hipGraph.  With set-up (wall clock "
+                  "to a device sync): plain = queue every level + compress(); batched = queue every "
+                  "level + create_batch (queue upload) + launch.",
     }
 
 
@@ -389,9 +475,12 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--parity-blocks", type=int, default=65536,
-                    help="seeded sample of blocks checked against the CPU oracle (0 = none)")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0 = min(16, cores))")
+    ap.add_argument("--parity-blocks", type=int, default=-1,
+                    help="blocks checked against the CPU oracle: -1 = every block (default), "
+                         "n > 0 = a seeded sample of n, 0 = none")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="oracle threads (0 = the host CPUs this process may use, at most 16)")
+    ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--batched-textures", type=int, default=300,
                     help="textures in the batched mip-chain workload (0 = skip it)")
     ap.add_argument("--batched-reps", type=int, default=3)
@@ -470,54 +559,55 @@ def main() -> int:
 
     out = buf.cpu().numpy().reshape(-1, 16)
 
-    # ---- parity and CPU baseline on a seeded block sample -------------------------
+    # ---- parity (every block, or a seeded sample) and the CPU baseline ----------
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
 
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    threads, host = cpu_threads(args.cpu_threads)
     parity = None
     cpu_baseline = None
     ops_per_block = None
-    if args.parity_blocks > 0:
-        n = min(args.parity_blocks, nblocks)
-        ids = np.sort(np.random.default_rng(7).choice(nblocks, n, replace=False))
-        log(f"bench: oracle on {n} sampled blocks with {threads} threads")
-        oracle_lib.compress_rgba8_sample(6, settings.to_bytes(), img, ids[:256], threads)  # warm
-        t1 = time.perf_counter()
-        ref = oracle_lib.compress_rgba8_sample(6, settings.to_bytes(), img, ids, threads).reshape(-1, 16)
-        cpu_s = time.perf_counter() - t1
-        got = out[ids]
-        same = (got == ref).all(axis=1)
-        # PSNR over the sampled blocks' pixels, GPU vs oracle blocks, both
-        # decoded by the oracle's decoder.
-        bw = W // 4
-        ys, xs = ids // bw, ids % bw
-        orig = np.stack([img[y * 4:y * 4 + 4, x * 4:x * 4 + 4] for y, x in zip(ys, xs)])
-        def dec(blocks):
-            return np.stack([oracle_lib.decode_block("bc7", b.tobytes(), 16, 64).reshape(4, 4, 4)
-                             for b in blocks])
-        n_psnr = min(n, 16384)
-        p_gpu = psnr_rgb(orig[:n_psnr], dec(got[:n_psnr]))
-        p_ref = psnr_rgb(orig[:n_psnr], dec(ref[:n_psnr]))
+    if args.parity_blocks != 0:
+        from metrics_ref import calculate_image_metrics_rgba8
+
+        if args.parity_blocks < 0 or args.parity_blocks >= nblocks:
+            log(f"bench: oracle on all {nblocks} blocks with {threads} threads")
+            t1 = time.perf_counter()
+            ref = oracle_lib.compress_rgba8(6, settings.to_bytes(), img, threads).reshape(-1, 16)
+            full_s = time.perf_counter() - t1
+            same = (out == ref).all(axis=1)
+            p_gpu = calculate_image_metrics_rgba8(img, oracle_lib.decompress_rgba8(6, W, H, out), 3)
+            p_ref = calculate_image_metrics_rgba8(img, oracle_lib.decompress_rgba8(6, W, H, ref), 3)
+            checked = nblocks
+        else:
+            full_s = None
+            ids = np.sort(np.random.default_rng(7).choice(nblocks, args.parity_blocks, replace=False))
+            ref = oracle_lib.compress_rgba8_sample(6, settings.to_bytes(), img, ids, threads).reshape(-1, 16)
+            same = (out[ids] == ref).all(axis=1)
+            bw = W // 4
+            sub = np.stack([img[(i // bw) * 4:(i // bw) * 4 + 4, (i % bw) * 4:(i % bw) * 4 + 4] for i in ids])
+            dec = lambda blocks: np.stack([oracle_lib.decode_block("bc7", b.tobytes(), 16, 64).reshape(4, 4, 4)
+                                           for b in blocks])
+            p_gpu = calculate_image_metrics_rgba8(sub, dec(out[ids]), 3)
+            p_ref = calculate_image_metrics_rgba8(sub, dec(ref), 3)
+            checked = int(ids.size)
         parity = {
-            "blocks_checked": int(n),
+            "blocks_checked": int(checked),
             "identical_frac": float(same.mean()),
-            "psnr_gpu_db": round(p_gpu, 4),
-            "psnr_ref_db": round(p_ref, 4),
-            "psnr_delta_db": round(p_gpu - p_ref, 6),
-            "psnr_blocks": int(n_psnr),
+            "psnr_gpu_db": round(p_gpu["overall_psnr"], 6),
+            "psnr_ref_db": round(p_ref["overall_psnr"], 6),
+            "psnr_delta_db": round(p_gpu["overall_psnr"] - p_ref["overall_psnr"], 9),
+            "psnr": "calculate_image_metrics_rgba8 (tests/metrics.rs:40-114), RGB in linear light, "
+                    "both block images decoded by the oracle's decoder",
         }
-        cpu_baseline = {
-            "value": round(n * 16 / cpu_s / 1e6, 3),
-            "unit": "Mpix/s",
-            "cores": threads,
-            "kind": "port",
-            "sample": f"{n} seeded random blocks of the same 4096x4096 opaque_basic workload "
-                      f"({n * 16} px), C++ oracle, OpenMP over blocks",
-        }
-        # Algorithmic op count per block (f32 ops as written in the reference,
-        # counted by the instrumented oracle build) on 1024 of the sampled blocks.
-        ops = oracle_lib.count_ops_rgba8(6, settings.to_bytes(), img, ids[:1024])
+        if full_s is not None:
+            parity["oracle_full_image_s"] = round(full_s, 3)
+            parity["oracle_full_image_threads"] = threads
+
+    if not args.no_cpu_baseline:
+        cpu_baseline = cpu_baseline_leg(img, settings, nblocks, threads, host)
+        ops = oracle_lib.count_ops_rgba8(6, settings.to_bytes(), img,
+                                         np.sort(np.random.default_rng(7).choice(nblocks, 1024, replace=False)))
         ops_per_block = ops / 1024.0
 
     roofline = None
