@@ -55,6 +55,7 @@ SIGNATURES = [
     ("bcn_decode_rgba16f", _i32, [_i32, _vp, _sz, _u32, _u32, _vp, _u32, _u32, _vp]),
     ("bcn_decode_rgba32f", _i32, [_i32, _vp, _sz, _u32, _u32, _vp, _u32, _u32, _vp]),
     ("bcn_bc6h_srgb8_thresholds", None, [_vp]),
+    ("bcn_bc6h_srgb8_buckets", _i32, [_vp]),
     ("bcn_dxgi_format", _u32, [_i32]),
     ("bcn_dds_data_size", _sz, [_u32, _u32, _u32, _u32, _u32]),
     ("bcn_dds_write_header", _i32, [_u32, _u32, _u32, _u32, _u32, _u32, _vp, _sz]),

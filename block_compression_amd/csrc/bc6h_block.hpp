@@ -29,7 +29,12 @@ struct Bc6hParams {
 
 // Endpoint field layout of the 14 modes, in bitstream order after the mode
 // bits: channel | endpoint << 2 | first_bit << 4 | bits << 9 | reversed << 15.
-BCN_TABLE uint16_t kBc6hFields[14][23] = {
+// One constexpr statement of the layout: the encoder's packer reads the
+// device copy kBc6hFields, the decoder a table derived from it.
+struct Bc6hFieldTable {
+  uint16_t f[14][23];
+};
+constexpr Bc6hFieldTable kBc6hFieldSrc = {{
     {0x0249, 0x024A, 0x024E, 0x1400, 0x1401, 0x1402, 0x0A04, 0x024D, 0x0809, 0x0A05, 0x020E, 0x080D, 0x0A06, 0x021E, 0x080A, 0x0A08, 0x022E, 0x0A0C, 0x023E},
     {0x0259, 0x024D, 0x025D, 0x0E00, 0x020E, 0x021E, 0x024A, 0x0E01, 0x025A, 0x022E, 0x0249, 0x0E02, 0x023E, 0x025E, 0x024E, 0x0C04, 0x0809, 0x0C05, 0x080D, 0x0C06, 0x080A, 0x0C08, 0x0C0C},
     {0x1400, 0x1401, 0x1402, 0x0A04, 0x02A0, 0x0809, 0x0805, 0x02A1, 0x020E, 0x080D, 0x0806, 0x02A2, 0x021E, 0x080A, 0x0A08, 0x022E, 0x0A0C, 0x023E},
@@ -44,14 +49,22 @@ BCN_TABLE uint16_t kBc6hFields[14][23] = {
     {0x1400, 0x1401, 0x1402, 0x1204, 0x02A0, 0x1205, 0x02A1, 0x1206, 0x02A2},
     {0x1400, 0x1401, 0x1402, 0x1004, 0x84A0, 0x1005, 0x84A1, 0x1006, 0x84A2},
     {0x1400, 0x1401, 0x1402, 0x0804, 0x8CA0, 0x0805, 0x8CA1, 0x0806, 0x8CA2},
-};
+}};
+BCN_TABLE Bc6hFieldTable kBc6hFields = kBc6hFieldSrc;
 BCN_TABLE uint8_t kBc6hFieldCount[14] = {19, 23, 18, 20, 20, 19, 19, 21, 21, 23, 6, 9, 9, 9};
 // Mode bits (bc6h.rs:131-135; 2 bits for modes 0 and 1, else 5).
 BCN_TABLE uint8_t kBc6hPrefix[14] = {0, 1, 2, 6, 10, 14, 18, 22, 26, 30, 3, 7, 11, 15};
-// Delta bits of R, G, B per mode (the format's per-mode precision table).
-BCN_TABLE uint8_t kBc6hDeltaBits[3][14] = {{5, 6, 5, 4, 4, 5, 6, 5, 5, 6, 10, 9, 8, 4},
-                                           {5, 6, 4, 5, 4, 5, 5, 6, 5, 6, 10, 9, 8, 4},
-                                           {5, 6, 4, 4, 5, 5, 5, 5, 6, 6, 10, 9, 8, 4}};
+// Base-endpoint bits and delta bits of R, G, B per mode (the format's
+// per-mode precision table), one constexpr statement for encoder and decoder.
+struct Bc6hPrecTable {
+  uint8_t base[14];
+  uint8_t delta[3][14];
+};
+constexpr Bc6hPrecTable kBc6hPrecSrc = {{10, 7, 11, 11, 11, 9, 8, 8, 8, 6, 10, 11, 12, 16},
+                                        {{5, 6, 5, 4, 4, 5, 6, 5, 5, 6, 10, 9, 8, 4},
+                                         {5, 6, 4, 5, 4, 5, 5, 6, 5, 6, 10, 9, 8, 4},
+                                         {5, 6, 4, 4, 5, 5, 5, 5, 6, 6, 10, 9, 8, 4}}};
+BCN_TABLE Bc6hPrecTable kBc6hPrec = kBc6hPrecSrc;
 
 // bc6h.rs:137-156 (constants as Rust evaluates them, in f32)
 BCN_HD float bc6h_span(int m) {
@@ -693,13 +706,13 @@ BCN_HD void bc6h_code(const Bc6hBest& best, uint32_t out[4]) {
 #pragma unroll
     for (int p = 0; p < 3; p++) {
       int v = q[e][p];
-      if (e > 0 && m != 9 && m != 10) v = (v - q[0][p]) & ((1 << kBc6hDeltaBits[p][m]) - 1);
+      if (e > 0 && m != 9 && m != 10) v = (v - q[0][p]) & ((1 << kBc6hPrec.delta[p][m]) - 1);
       f[e * 3 + p] = (uint32_t)v;
     }
   BitWriter w;
   w.put(m <= 1 ? 2 : 5, kBc6hPrefix[m]);
   for (int i = 0; i < kBc6hFieldCount[m]; i++) {
-    uint32_t t = kBc6hFields[m][i];
+    uint32_t t = kBc6hFields.f[m][i];
     uint32_t c = t & 3u, e = (t >> 2) & 3u, first = (t >> 4) & 31u, n = (t >> 9) & 63u;
     uint32_t slot = e * 3 + c, v = 0;
 #pragma unroll

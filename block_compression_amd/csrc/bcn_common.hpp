@@ -111,6 +111,25 @@ BCN_HD uint32_t pk_wdelta16(uint32_t w, uint32_t d) {
 #endif
 }
 
+// Lane-wise logical shifts of unsigned 16-bit lanes by per-lane amounts
+// (v_pk_lshlrev_b16 / v_pk_lshrrev_b16); amounts below 16.
+BCN_HD uint32_t pk_shl16(uint32_t v, uint32_t sh) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+  return __builtin_bit_cast(uint32_t, (u16x2)(__builtin_bit_cast(u16x2, v) << __builtin_bit_cast(u16x2, sh)));
+#else
+  return pk16((int)(((v & 0xFFFFu) << (sh & 0xFu)) & 0xFFFFu), (int)(((v >> 16) << ((sh >> 16) & 0xFu)) & 0xFFFFu));
+#endif
+}
+BCN_HD uint32_t pk_shr16(uint32_t v, uint32_t sh) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+  return __builtin_bit_cast(uint32_t, (u16x2)(__builtin_bit_cast(u16x2, v) >> __builtin_bit_cast(u16x2, sh)));
+#else
+  return pk16((int)((v & 0xFFFFu) >> (sh & 0xFu)), (int)((v >> 16) >> ((sh >> 16) & 0xFu)));
+#endif
+}
+
 // Byte permute (v_perm_b32): byte i of the result is byte sel.byte_i of
 // {src1 (0..3), src0 (4..7)}; selector 12 gives 0.
 BCN_HD uint32_t perm_b32(uint32_t src0, uint32_t src1, uint32_t sel) {
@@ -125,6 +144,16 @@ BCN_HD uint32_t perm_b32(uint32_t src0, uint32_t src1, uint32_t sel) {
     r |= b << (8 * i);
   }
   return r;
+#endif
+}
+
+// Bits off .. off + w - 1 of x (v_bfe_u32), for off < 32 and off + w <= 32;
+// w = 0 gives 0.
+BCN_HD uint32_t bfe_u32(uint32_t x, uint32_t off, uint32_t w) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_ubfe(x, off, w);
+#else
+  return w == 0 ? 0u : (x >> off) & (w >= 32 ? 0xFFFFFFFFu : ((1u << w) - 1u));
 #endif
 }
 
@@ -426,6 +455,16 @@ BCN_HD uint32_t pk_mad16(uint32_t a, uint32_t b, uint32_t c) {
   return as_u32(as_s16x2(a) * as_s16x2(b) + as_s16x2(c));
 #else
   return pk16(lane16(a, 0) * lane16(b, 0) + lane16(c, 0), lane16(a, 1) * lane16(b, 1) + lane16(c, 1));
+#endif
+}
+
+// Lane-wise a * w.lo + c modulo 2^16: both lanes of a times the low lane of
+// w (v_pk_mad_u16 with the low half of w read for both lanes).
+BCN_HD uint32_t pk_mad16_lo(uint32_t a, uint32_t w, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return as_u32(as_s16x2(a) * as_s16x2(w).xx + as_s16x2(c));
+#else
+  return pk16(lane16(a, 0) * lane16(w, 0) + lane16(c, 0), lane16(a, 1) * lane16(w, 0) + lane16(c, 1));
 #endif
 }
 

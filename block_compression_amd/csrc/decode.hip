@@ -8,6 +8,7 @@
 // adjacent lanes write adjacent segments of the same rows.  HBM-bound: 80
 // bytes of traffic per BC7 block (16 in, 64 out).
 #include <cmath>
+#include <mutex>
 
 #include "decode_block.hpp"
 #include "bcn_internal.hpp"
@@ -15,6 +16,10 @@
 namespace bcn {
 
 constexpr int kDecGroup = 256;
+
+// The sRGB8 bucket table (srgb8_word) on each device, uploaded once per
+// device before its first BC6H -> RGBA8 decode.
+__device__ __align__(16) uint16_t g_srgb8_buckets[kSrgb8Buckets];
 
 // Output forms of the BC6H kernel.
 enum Bc6hOut : int {
@@ -26,49 +31,22 @@ enum Bc6hOut : int {
 
 template <int FMT>
 __global__ __launch_bounds__(kDecGroup) void decode_rgba8_kernel(const uint8_t* __restrict__ blocks,
-                                                                 uint32_t bw, uint32_t nblocks,
+                                                                 BlockRows rows, uint32_t nblocks,
                                                                  uint8_t* __restrict__ rgba,
                                                                  uint32_t pitch) {
-  uint32_t idx = blockIdx.x * kDecGroup + threadIdx.x;
-  if (idx >= nblocks) return;
   constexpr int BB = (FMT == BCN_BC1 || FMT == BCN_BC4) ? 8 : 16;
-  uint32_t in[4] = {0, 0, 0, 0};
+  const uint32_t idx = blockIdx.x * kDecGroup + threadIdx.x;
+  if (idx >= nblocks) return;
   const uint32_t* src = reinterpret_cast<const uint32_t*>(blocks + (size_t)idx * BB);
+  uint32_t in[4];
 #pragma unroll
-  for (int i = 0; i < BB / 4; i++) in[i] = src[i];
+  for (int i = 0; i < 4; i++) in[i] = i < BB / 4 ? src[i] : 0u;
   uint32_t px[16];
-  if (FMT == BCN_BC1) {
-    decode_color(in[0], in[1], false, px);
-  } else if (FMT == BCN_BC2 || FMT == BCN_BC3) {
-    decode_color(in[2], in[3], true, px);
-    if (FMT == BCN_BC2) {
-      // decode_sharp_alpha_block (block.rs:200-217): 4-bit alpha x 17
-#pragma unroll
-      for (int k = 0; k < 16; k++) {
-        uint32_t a = ((k < 8 ? in[0] : in[1]) >> (4 * (k & 7))) & 15u;
-        px[k] = (px[k] & 0x00FFFFFFu) | ((a * 17u) << 24);
-      }
-    } else {
-      uint8_t a[16];
-      decode_channel(in[0], in[1], a);
-#pragma unroll
-      for (int k = 0; k < 16; k++) px[k] = (px[k] & 0x00FFFFFFu) | ((uint32_t)a[k] << 24);
-    }
-  } else if (FMT == BCN_BC4) {
-    uint8_t r[16];
-    decode_channel(in[0], in[1], r);
-#pragma unroll
-    for (int k = 0; k < 16; k++) px[k] = r[k];  // (R, 0, 0, 0) (decode.rs:97-118)
-  } else if (FMT == BCN_BC5) {
-    uint8_t r[16], g[16];
-    decode_channel(in[0], in[1], r);
-    decode_channel(in[2], in[3], g);
-#pragma unroll
-    for (int k = 0; k < 16; k++) px[k] = r[k] | ((uint32_t)g[k] << 8);  // (R, G, 0, 0)
-  } else {
-    decode_bc7(in, px);
-  }
-  uint32_t yy = idx / bw, xx = idx - yy * bw;
+  // BC7's three subset records per lane, interleaved by lane (12 KiB).
+  __shared__ Bc7SubsetData subs[FMT == BCN_BC7 ? 3 * kDecGroup : 1];
+  decode_rgba8_block<FMT>(in, px, subs + (FMT == BCN_BC7 ? threadIdx.x : 0), kDecGroup);
+  uint32_t yy, xx;
+  block_pos(rows, idx, yy, xx);
   uint8_t* o = rgba + (size_t)yy * 4 * pitch + (size_t)xx * 16;
 #pragma unroll
   for (int y = 0; y < 4; y++)
@@ -76,73 +54,144 @@ __global__ __launch_bounds__(kDecGroup) void decode_rgba8_kernel(const uint8_t* 
         make_uint4(px[y * 4 + 0], px[y * 4 + 1], px[y * 4 + 2], px[y * 4 + 3]);
 }
 
+// The BC6H decoders' LDS: the decoder's field table (one copy per
+// workgroup) and each lane's 18-word scratch row, interleaved by lane.
+struct Bc6hLds {
+  uint16_t fields[kBc6hDecRows * 23];
+  uint32_t scratch[18 * kDecGroup];
+};
+
+__device__ __forceinline__ void load_bc6h_fields(Bc6hLds& l) {
+  for (uint32_t i = threadIdx.x; i < kBc6hDecRows * 23; i += kDecGroup) l.fields[i] = (&kBc6hDec.f[0][0])[i];
+}
+
 // BC6H -> RGBA8 (decode.rs:161-181): unsigned decode, linear_to_srgb per
 // channel, alpha 0.  SGN: signed blocks, whose negative values (and -0) map
-// to code 0 as linear_to_srgb clamps them.
+// to code 0 as linear_to_srgb clamps them.  Each workgroup copies the sRGB8
+// bucket table (7,936 bytes) to LDS; a channel is then one LDS read.
 template <bool SGN>
 __global__ __launch_bounds__(kDecGroup) void decode_bc6h_srgb8_kernel(
-    const uint8_t* __restrict__ blocks, uint32_t bw, uint32_t nblocks, uint8_t* __restrict__ rgba,
-    uint32_t pitch, Srgb8Table tab) {
-  __shared__ uint16_t lt[256];
-  lt[threadIdx.x] = tab.t[threadIdx.x];
+    const uint8_t* __restrict__ blocks, BlockRows rows, uint32_t nblocks, uint8_t* __restrict__ rgba,
+    uint32_t pitch) {
+  __shared__ __align__(16) uint16_t lb[kSrgb8Buckets];
+  __shared__ Bc6hLds lds;
+  for (uint32_t i = threadIdx.x; i < kSrgb8Buckets / 8; i += kDecGroup)
+    reinterpret_cast<uint4*>(lb)[i] = reinterpret_cast<const uint4*>(g_srgb8_buckets)[i];
+  load_bc6h_fields(lds);
   __syncthreads();
-  uint32_t idx = blockIdx.x * kDecGroup + threadIdx.x;
+  const uint32_t idx = blockIdx.x * kDecGroup + threadIdx.x;
   if (idx >= nblocks) return;
   const uint32_t* src = reinterpret_cast<const uint32_t*>(blocks + (size_t)idx * 16);
-  uint32_t in[4] = {src[0], src[1], src[2], src[3]};
-  uint16_t h[48];
-  decode_bc6h_t<SGN>(in, h);
-  uint32_t yy = idx / bw, xx = idx - yy * bw;
+  const uint32_t in[4] = {src[0], src[1], src[2], src[3]};
+  const Bc6hScratch sc{lds.scratch + threadIdx.x, kDecGroup};
+  Bc6hIdx st = decode_bc6h_header<SGN>(in, sc, lds.fields);
+  uint32_t yy, xx;
+  block_pos(rows, idx, yy, xx);
   uint8_t* o = rgba + (size_t)yy * 4 * pitch + (size_t)xx * 16;
-  // Negative values (and -0) code as 0, the code of +0: select the input.
-  auto code = [&](uint16_t v) -> uint32_t {
-    return srgb8_code((SGN && (v & 0x8000u)) ? (uint16_t)0 : v, lt);
+  // Negative values (and -0) code as 0, the code of +0.
+  auto word = [&](uint16_t v) -> uint32_t {
+    const uint32_t x = SGN ? (uint32_t)max((int)(int16_t)v, 0) : (uint32_t)v;
+    return srgb8_word(x, lb);
   };
+  // Row by row: each row is stored as soon as it is decoded.
 #pragma unroll
   for (int y = 0; y < 4; y++) {
     uint32_t px[4];
 #pragma unroll
     for (int x = 0; x < 4; x++) {
-      int k = y * 4 + x;
-      px[x] = code(h[k * 3]) | (code(h[k * 3 + 1]) << 8) | (code(h[k * 3 + 2]) << 16);
+      uint16_t h[3];
+      decode_bc6h_texel<SGN>(st, y * 4 + x, sc, h);
+      // The high bytes of the three words: R, G, then B; alpha 0.
+      const uint32_t rg = perm_b32(word(h[1]), word(h[0]), 0x0C0C0501u);
+      px[x] = perm_b32(word(h[2]), rg, 0x0C050100u);
     }
     *reinterpret_cast<uint4*>(o + (size_t)y * pitch) = make_uint4(px[0], px[1], px[2], px[3]);
   }
 }
 
+// One block row per lane (N x 16 bytes, lane order) stored through the
+// wave's LDS slice, when the wave's 64 block rows are consecutive in memory
+// from `dst`: store instruction x then writes bytes x * 1024 .. x * 1024 +
+// 1023 of the wave's run (lane L the 16 bytes at x * 1024 + 16 L), instead of
+// 16 bytes every N * 16 bytes.  Element e = 64 x + L of the run is chunk
+// e % N of lane e / N's row, which the slice holds at index e.
+template <int N>
+__device__ __forceinline__ void store_row_run(const uint4 (&v)[N], uint4* slice, uint8_t* dst,
+                                              uint32_t lane) {
+#pragma unroll
+  for (int c = 0; c < N; c++) slice[lane * N + c] = v[c];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int x = 0; x < N; x++)
+    reinterpret_cast<uint4*>(dst)[x * 64 + lane] = slice[x * 64 + lane];
+  // The slice is rewritten by the next row only after every lane has read it.
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int OUT, bool SGN>
 __global__ __launch_bounds__(kDecGroup) void decode_bc6h_kernel(const uint8_t* __restrict__ blocks,
-                                                                uint32_t bw, uint32_t nblocks,
+                                                                BlockRows rows, uint32_t nblocks,
                                                                 uint8_t* __restrict__ rgba,
                                                                 uint32_t pitch) {
-  uint32_t idx = blockIdx.x * kDecGroup + threadIdx.x;
+  __shared__ Bc6hLds lds;
+  // RGBA outputs: each wave's row-run slice, 64 rows of 32 (16F) or 64 (32F)
+  // bytes.
+  constexpr bool F32 = OUT == kOutF32Rgba || OUT == kOutF32RefRgb;
+  constexpr bool kRun = OUT == kOutF16Rgba || OUT == kOutF32Rgba;
+  constexpr int N = F32 ? 4 : 2;  // 16-byte chunks per block row
+  __shared__ uint4 runs[kRun ? (kDecGroup / 64) * 64 * N : 1];
+  load_bc6h_fields(lds);
+  __syncthreads();
+  const uint32_t idx = blockIdx.x * kDecGroup + threadIdx.x;
   if (idx >= nblocks) return;
   const uint32_t* src = reinterpret_cast<const uint32_t*>(blocks + (size_t)idx * 16);
-  uint32_t in[4] = {src[0], src[1], src[2], src[3]};
-  uint16_t h[48];
-  decode_bc6h_t<SGN>(in, h);
-  constexpr bool F32 = OUT == kOutF32Rgba || OUT == kOutF32RefRgb;
-  uint32_t yy = idx / bw, xx = idx - yy * bw;
+  const uint32_t in[4] = {src[0], src[1], src[2], src[3]};
+  const Bc6hScratch sc{lds.scratch + threadIdx.x, kDecGroup};
+  Bc6hIdx st = decode_bc6h_header<SGN>(in, sc, lds.fields);
+  uint32_t yy, xx;
+  block_pos(rows, idx, yy, xx);
   uint8_t* o = rgba + (size_t)yy * 4 * pitch + (size_t)xx * (F32 ? 64 : 32);
+  // Whether the wave's 64 blocks all exist and sit in one block row: then its
+  // rows are consecutive in memory and go out through the row-run slice.
+  const uint32_t lane = threadIdx.x & 63u, first = idx - lane;
+  uint32_t fy, fx;
+  block_pos(rows, first, fy, fx);
+  const bool run = kRun && first + 63 < nblocks && fx + 63 < rows.bw;
+  uint4* slice = runs + (threadIdx.x >> 6) * 64 * N;
+  uint8_t* run_dst = rgba + (size_t)fy * 4 * pitch + (size_t)fx * (F32 ? 64 : 32);
+  // Row by row: each row is stored as soon as it is decoded.
 #pragma unroll
   for (int y = 0; y < 4; y++) {
     uint8_t* r = o + (size_t)y * pitch;
-    const uint16_t* hr = h + y * 12;  // the row's 4 texels, RGB interleaved
-    if (OUT == kOutF16Rgba) {
-      uint4* row = reinterpret_cast<uint4*>(r);
+    uint16_t hr[12];  // the row's 4 texels, RGB interleaved
 #pragma unroll
-      for (int x = 0; x < 4; x += 2)
-        row[x / 2] = make_uint4((uint32_t)hr[x * 3] | ((uint32_t)hr[x * 3 + 1] << 16),
-                                (uint32_t)hr[x * 3 + 2] | (0x3C00u << 16),
-                                (uint32_t)hr[x * 3 + 3] | ((uint32_t)hr[x * 3 + 4] << 16),
-                                (uint32_t)hr[x * 3 + 5] | (0x3C00u << 16));
-    } else if (OUT == kOutF32Rgba) {
-      uint4* row = reinterpret_cast<uint4*>(r);
+    for (int x = 0; x < 4; x++) decode_bc6h_texel<SGN>(st, y * 4 + x, sc, hr + 3 * x);
+    if (kRun) {
+      uint4 v[N];
 #pragma unroll
-      for (int x = 0; x < 4; x++)
-        row[x] = make_uint4(__float_as_uint(half_to_float(hr[x * 3])),
-                            __float_as_uint(half_to_float(hr[x * 3 + 1])),
-                            __float_as_uint(half_to_float(hr[x * 3 + 2])), 0x3F800000u);
+      for (int x = 0; x < 4; x++) {
+        if (OUT == kOutF16Rgba) {
+          const uint2 t = make_uint2((uint32_t)hr[x * 3] | ((uint32_t)hr[x * 3 + 1] << 16),
+                                     (uint32_t)hr[x * 3 + 2] | (0x3C00u << 16));
+          if (x & 1) v[x / 2].z = t.x, v[x / 2].w = t.y;
+          else v[x / 2].x = t.x, v[x / 2].y = t.y;
+        } else {
+          v[x % N] = make_uint4(__float_as_uint(half_to_float(hr[x * 3])),
+                                __float_as_uint(half_to_float(hr[x * 3 + 1])),
+                                __float_as_uint(half_to_float(hr[x * 3 + 2])), 0x3F800000u);
+        }
+      }
+      if (run) {
+        store_row_run<N>(v, slice, run_dst + (size_t)y * pitch, lane);
+      } else {
+        uint4* row = reinterpret_cast<uint4*>(r);
+#pragma unroll
+        for (int c = 0; c < N; c++) row[c] = v[c];
+      }
     } else if (OUT == kOutF16RefRgb) {
       // decode_block_bc6h writes texel j at j*3 (block.rs:753) from the
       // block's RGBA16F offset (decode.rs:271-279): 12 halves, the other 4
@@ -183,8 +232,12 @@ uint8_t linear_to_srgb8_host(float lin) {
   return (uint8_t)roundf(v * 255.0f);
 }
 
-// Thresholds over every unsigned BC6H output (bit patterns 0..0x7BFF; the
-// unsigned finish_unquantize, block.rs:1391-1397, never exceeds 0x7BFF).
+// sRGB8 thresholds over every unsigned BC6H output (bit patterns 0..0x7BFF;
+// the unsigned finish_unquantize, block.rs:1391-1397, never exceeds 0x7BFF):
+// t[k] is the smallest pattern whose code is >= k (t[0] = 0).
+struct Srgb8Table {
+  uint16_t t[256];
+};
 const Srgb8Table& srgb8_table() {
   static const Srgb8Table tab = [] {
     Srgb8Table t;
@@ -200,6 +253,63 @@ const Srgb8Table& srgb8_table() {
   return tab;
 }
 
+// The bucket entries of srgb8_word from the thresholds; ok is false if some
+// bucket of 8 patterns held two rises (the entries could not express it).
+struct Srgb8Buckets {
+  uint16_t e[kSrgb8Buckets];
+  bool ok;
+};
+const Srgb8Buckets& srgb8_buckets() {
+  static const Srgb8Buckets tab = [] {
+    const Srgb8Table& t = srgb8_table();
+    auto code = [&](uint32_t h) {
+      int c = 0;
+      for (int k = 1; k < 256; k++) c += h >= t.t[k] ? 1 : 0;
+      return c;
+    };
+    Srgb8Buckets b;
+    b.ok = true;
+    for (uint32_t i = 0; i < kSrgb8Buckets; i++) {
+      const int base = code(8 * i);
+      uint32_t off = 8;
+      for (uint32_t j = 1; j < 8; j++) {
+        const int c = code(8 * i + j);
+        if (c != base && off == 8) off = j;
+        if (c != base + (off <= j ? 1 : 0)) b.ok = false;
+      }
+      b.e[i] = (uint16_t)(256 * base + 32 * (8 - off));
+    }
+    return b;
+  }();
+  return tab;
+}
+
+// Uploads the bucket table to the current device, once per device (a
+// synchronous copy at the first BC6H -> RGBA8 decode there).
+int srgb8_buckets_upload() {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static bool done[kMaxDevices] = {};
+  const Srgb8Buckets& b = srgb8_buckets();
+  if (!b.ok) return BCN_ERR_UNSUPPORTED;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return BCN_ERR_HIP;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!done[dev]) {
+    if (hipMemcpyToSymbol(HIP_SYMBOL(g_srgb8_buckets), b.e, sizeof(b.e), 0, hipMemcpyHostToDevice) != hipSuccess)
+      return BCN_ERR_HIP;
+    done[dev] = true;
+  }
+  return BCN_OK;
+}
+
+// Block grid of an image: bw x rows blocks, both below 2^24 (block_pos) and
+// at most 2^31 blocks in all.
+bool block_grid_ok(uint32_t width, uint32_t height) {
+  const uint64_t bw = width / 4, bh = height / 4;
+  return bw < (1u << 24) && bh < (1u << 24) && bw * bh <= (1ull << 31);
+}
+
 bool launch_ok() { return hipGetLastError() == hipSuccess; }
 
 // decompress_blocks_as_rgba16f / _rgba32f (decode.rs:397-468): BC6H only
@@ -212,6 +322,7 @@ int decode_bc6h_image(bcn_variant variant, const void* blocks, size_t blocks_siz
   if (variant == BCN_BC6H_SF16) is_signed = 1;
   if (flags & ~(uint32_t)BCN_DECODE_REFERENCE_LAYOUT) return BCN_ERR_INVALID_ARGUMENT;
   if (width % 4 || height % 4) return BCN_ERR_NOT_MULTIPLE_OF_4;
+  if (!block_grid_ok(width, height)) return BCN_ERR_INVALID_ARGUMENT;
   uint32_t bw = width / 4, nblocks = bw * (height / 4);
   if (blocks_size != (size_t)nblocks * 16) return BCN_ERR_BUFFER_TOO_SMALL;
   if ((uint64_t)rgba_pitch < (uint64_t)width * (f32 ? 16 : 8)) return BCN_ERR_INVALID_ARGUMENT;
@@ -219,12 +330,13 @@ int decode_bc6h_image(bcn_variant variant, const void* blocks, size_t blocks_siz
   if (!blocks || !rgba) return BCN_ERR_INVALID_ARGUMENT;
   if ((((uintptr_t)rgba | rgba_pitch) & 15u) || ((uintptr_t)blocks & 3u)) return BCN_ERR_MISALIGNED;
   dim3 grid((nblocks + kDecGroup - 1) / kDecGroup), block(kDecGroup);
+  const BlockRows rows = block_rows(bw);
   hipStream_t s = static_cast<hipStream_t>(hip_stream);
   const uint8_t* b = static_cast<const uint8_t*>(blocks);
   uint8_t* o = static_cast<uint8_t*>(rgba);
   bool ref = flags & BCN_DECODE_REFERENCE_LAYOUT;
   int out = f32 ? (ref ? kOutF32RefRgb : kOutF32Rgba) : (ref ? kOutF16RefRgb : kOutF16Rgba);
-#define BCN_DEC6(O, S) hipLaunchKernelGGL((decode_bc6h_kernel<O, S>), grid, block, 0, s, b, bw, nblocks, o, rgba_pitch)
+#define BCN_DEC6(O, S) hipLaunchKernelGGL((decode_bc6h_kernel<O, S>), grid, block, 0, s, b, rows, nblocks, o, rgba_pitch)
   if (is_signed) {
     switch (out) {
       case kOutF16Rgba: BCN_DEC6(kOutF16Rgba, true); break;
@@ -254,6 +366,7 @@ int bcn_decode_rgba8(bcn_variant variant, const void* blocks, size_t blocks_size
                      uint32_t height, void* rgba, uint32_t rgba_pitch, void* hip_stream) {
   if (variant < BCN_BC1 || variant > BCN_BC6H_SF16) return BCN_ERR_INVALID_ARGUMENT;
   if (width % 4 || height % 4) return BCN_ERR_NOT_MULTIPLE_OF_4;
+  if (!block_grid_ok(width, height)) return BCN_ERR_INVALID_ARGUMENT;
   size_t bb = (variant == BCN_BC1 || variant == BCN_BC4) ? 8 : 16;
   uint32_t bw = width / 4, nblocks = bw * (height / 4);
   // decode.rs:343-355: the block slice and the image must have the exact sizes.
@@ -263,18 +376,23 @@ int bcn_decode_rgba8(bcn_variant variant, const void* blocks, size_t blocks_size
   if (!blocks || !rgba) return BCN_ERR_INVALID_ARGUMENT;
   if ((((uintptr_t)rgba | rgba_pitch) & 15u) || ((uintptr_t)blocks & 7u)) return BCN_ERR_MISALIGNED;
   dim3 grid((nblocks + kDecGroup - 1) / kDecGroup), block(kDecGroup);
+  const BlockRows rows = block_rows(bw);
   hipStream_t s = static_cast<hipStream_t>(hip_stream);
   const uint8_t* b = static_cast<const uint8_t*>(blocks);
   uint8_t* o = static_cast<uint8_t*>(rgba);
+  if (variant == BCN_BC6H || variant == BCN_BC6H_SF16) {
+    const int rc = srgb8_buckets_upload();
+    if (rc != BCN_OK) return rc;
+  }
   switch (variant) {
-    case BCN_BC1: hipLaunchKernelGGL(decode_rgba8_kernel<BCN_BC1>, grid, block, 0, s, b, bw, nblocks, o, rgba_pitch); break;
-    case BCN_BC2: hipLaunchKernelGGL(decode_rgba8_kernel<BCN_BC2>, grid, block, 0, s, b, bw, nblocks, o, rgba_pitch); break;
-    case BCN_BC3: hipLaunchKernelGGL(decode_rgba8_kernel<BCN_BC3>, grid, block, 0, s, b, bw, nblocks, o, rgba_pitch); break;
-    case BCN_BC4: hipLaunchKernelGGL(decode_rgba8_kernel<BCN_BC4>, grid, block, 0, s, b, bw, nblocks, o, rgba_pitch); break;
-    case BCN_BC5: hipLaunchKernelGGL(decode_rgba8_kernel<BCN_BC5>, grid, block, 0, s, b, bw, nblocks, o, rgba_pitch); break;
-    case BCN_BC6H: hipLaunchKernelGGL(decode_bc6h_srgb8_kernel<false>, grid, block, 0, s, b, bw, nblocks, o, rgba_pitch, srgb8_table()); break;
-    case BCN_BC6H_SF16: hipLaunchKernelGGL(decode_bc6h_srgb8_kernel<true>, grid, block, 0, s, b, bw, nblocks, o, rgba_pitch, srgb8_table()); break;
-    default: hipLaunchKernelGGL(decode_rgba8_kernel<BCN_BC7>, grid, block, 0, s, b, bw, nblocks, o, rgba_pitch); break;
+    case BCN_BC1: hipLaunchKernelGGL(decode_rgba8_kernel<BCN_BC1>, grid, block, 0, s, b, rows, nblocks, o, rgba_pitch); break;
+    case BCN_BC2: hipLaunchKernelGGL(decode_rgba8_kernel<BCN_BC2>, grid, block, 0, s, b, rows, nblocks, o, rgba_pitch); break;
+    case BCN_BC3: hipLaunchKernelGGL(decode_rgba8_kernel<BCN_BC3>, grid, block, 0, s, b, rows, nblocks, o, rgba_pitch); break;
+    case BCN_BC4: hipLaunchKernelGGL(decode_rgba8_kernel<BCN_BC4>, grid, block, 0, s, b, rows, nblocks, o, rgba_pitch); break;
+    case BCN_BC5: hipLaunchKernelGGL(decode_rgba8_kernel<BCN_BC5>, grid, block, 0, s, b, rows, nblocks, o, rgba_pitch); break;
+    case BCN_BC6H: hipLaunchKernelGGL(decode_bc6h_srgb8_kernel<false>, grid, block, 0, s, b, rows, nblocks, o, rgba_pitch); break;
+    case BCN_BC6H_SF16: hipLaunchKernelGGL(decode_bc6h_srgb8_kernel<true>, grid, block, 0, s, b, rows, nblocks, o, rgba_pitch); break;
+    default: hipLaunchKernelGGL(decode_rgba8_kernel<BCN_BC7>, grid, block, 0, s, b, rows, nblocks, o, rgba_pitch); break;
   }
   return launch_ok() ? BCN_OK : BCN_ERR_HIP;
 }
@@ -303,6 +421,13 @@ int bcn_decode_bc6h_rgba16f(const void* blocks, size_t blocks_size, uint32_t wid
 void bcn_bc6h_srgb8_thresholds(uint16_t* out) {
   const Srgb8Table& t = srgb8_table();
   for (int k = 0; k < 256; k++) out[k] = t.t[k];
+}
+
+int bcn_bc6h_srgb8_buckets(uint16_t* out) {
+  const Srgb8Buckets& b = srgb8_buckets();
+  if (!out) return BCN_ERR_INVALID_ARGUMENT;
+  for (uint32_t i = 0; i < kSrgb8Buckets; i++) out[i] = b.e[i];
+  return b.ok ? BCN_OK : BCN_ERR_UNSUPPORTED;
 }
 
 }  // extern "C"

@@ -71,27 +71,39 @@ BCN_HD void decode_color(uint32_t w0, uint32_t w1, bool opaque, uint32_t px[16])
   }
 }
 
-// 16 values of an 8-level alpha / BC4 channel block (block.rs:219-258).
-BCN_HD void decode_channel(uint32_t w0, uint32_t w1, uint8_t v[16]) {
-  uint32_t a0 = w0 & 0xFFu, a1 = (w0 >> 8) & 0xFFu;
-  uint32_t a[8];
-  a[0] = a0;
-  a[1] = a1;
-  if (a0 > a1) {
-    for (int i = 1; i < 7; i++) a[1 + i] = ((7 - i) * a0 + i * a1) / 7;
-  } else {
-    for (int i = 1; i < 5; i++) a[1 + i] = ((5 - i) * a0 + i * a1) / 5;
-    a[6] = 0;
-    a[7] = 255;
-  }
-  uint64_t idx = ((uint64_t)w0 >> 16) | ((uint64_t)w1 << 16);
+// Byte 2 of x and of y in bytes 0 and 1 (the rest 0).
+BCN_HD uint32_t bytes2(uint32_t x, uint32_t y) { return perm_b32(y, x, 0x0C0C0602u); }
+
+// The 16 values of an 8-level alpha / BC4 channel block (block.rs:219-258)
+// as four words of bytes: byte t of v4[j] is texel 4 j + t.
+//   * The palette is two words of bytes (levels 0..3 and 4..7), and one
+//     v_perm_b32 looks up four texels: its selector bytes are their 3-bit
+//     indices.
+//   * The interpolated levels ((n - i) a0 + i a1) / n, n = 7 or 5, equal
+//     (v * ceil(2^16 / n)) >> 16 for v <= 255 n (the rounding-up error,
+//     v * 5 / (7 * 2^16) or v * 4 / (5 * 2^16), stays below 1/n): byte 2 of
+//     a 24-bit multiply-add, which the packing perms take directly.
+BCN_HD void decode_channel4(uint32_t w0, uint32_t w1, uint32_t v4[4]) {
+  const int a0 = (int)(w0 & 0xFFu), a1 = (int)((w0 >> 8) & 0xFFu);
+  constexpr int k7 = 9363, k5 = 13108;  // ceil(2^16 / 7), ceil(2^16 / 5)
+  auto q7 = [&](int i) { return (uint32_t)(mul24(a0, (7 - i) * k7) + mul24(a1, i * k7)); };
+  auto q5 = [&](int i) { return (uint32_t)(mul24(a0, (5 - i) * k5) + mul24(a1, i * k5)); };
+  // Levels 0..3 = a0, a1, q(1), q(2); 4..7 = q(3), q(4) and q(5), q(6) or 0, 255.
+  const uint32_t lo7 = perm_b32(bytes2(q7(1), q7(2)), w0, 0x05040100u);
+  const uint32_t hi7 = perm_b32(bytes2(q7(5), q7(6)), bytes2(q7(3), q7(4)), 0x05040100u);
+  const uint32_t lo5 = perm_b32(bytes2(q5(1), q5(2)), w0, 0x05040100u);
+  const uint32_t hi5 = bytes2(q5(3), q5(4)) | 0xFF000000u;
+  const bool six = a0 > a1;
+  const uint32_t lo = six ? lo7 : lo5, hi = six ? hi7 : hi5;
+  // 48 index bits from bit 16 of the block; four 3-bit fields (12 bits) per
+  // word, spread to one field per byte in two steps (6-bit halves to 16-bit
+  // lanes, then 3-bit fields to bytes).
 #pragma unroll
-  for (int k = 0; k < 16; k++) {
-    uint32_t i = (uint32_t)(idx >> (3 * k)) & 7u;
-    uint32_t r = a[0];
-#pragma unroll
-    for (int t = 1; t < 8; t++) r = i == (uint32_t)t ? a[t] : r;
-    v[k] = (uint8_t)r;
+  for (int j = 0; j < 4; j++) {
+    const uint32_t x = j == 0 ? w0 >> 16 : (j == 1 ? funnel_shr(w1, w0, 28) : w1 >> (12 * j - 16));
+    const uint32_t y = bfe_u32(x, 0, 6) | (bfe_u32(x, 6, 6) << 16);
+    const uint32_t sel = (y & 0x00070007u) | ((y << 5) & 0x07000700u);
+    v4[j] = perm_b32(hi, lo, sel);
   }
 }
 
@@ -127,125 +139,200 @@ BCN_HD int bptc_weight_mul(int bits) { return bits == 2 ? 1365 : (bits == 3 ? 58
 BCN_HD int bptc_weight_rt(int bits, int i) { return (mul24(i, bptc_weight_mul(bits)) + 32) >> 6; }
 
 
-// Two 16-bit lanes of (a * (64 - w) + b * w + 32) >> 6 (block.rs:1256-1258):
-// lo/hi/w hold one channel per half-word.  Every term stays below 2^16
-// (255 * 64 + 32), so packed 16-bit integer arithmetic is exact.
-BCN_HD uint32_t interp2(uint32_t lo, uint32_t hi, uint32_t w) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
-  const u16x2 a = __builtin_bit_cast(u16x2, lo), b = __builtin_bit_cast(u16x2, hi);
-  const u16x2 ww = __builtin_bit_cast(u16x2, w);
-  const u16x2 c64 = {64, 64}, c32 = {32, 32}, c6 = {6, 6};
-  const u16x2 r = (a * (c64 - ww) + b * ww + c32) >> c6;
-  return __builtin_bit_cast(uint32_t, r);
-#else
-  uint32_t r = 0;
-  for (int h = 0; h < 2; h++) {
-    uint32_t a = (lo >> (16 * h)) & 0xFFFFu, b = (hi >> (16 * h)) & 0xFFFFu;
-    uint32_t ww = (w >> (16 * h)) & 0xFFFFu;
-    r |= (((a * (64 - ww) + b * ww + 32) >> 6) & 0xFFFFu) << (16 * h);
-  }
-  return r;
-#endif
+// Per-mode fields as nibble tables, mode m in bits 4m..4m+3 (kBc7Modes in
+// that order): one v_bfe_u32 per field instead of a per-lane table load.
+constexpr uint32_t bc7_nibbles(uint8_t Bc7ModeInfo::*f) {
+  uint32_t k = 0;
+  const Bc7ModeInfo t[8] = {{3, 4, 0, 0, 4, 0, 1, 3, 0}, {2, 6, 0, 0, 6, 0, 2, 3, 0}, {3, 6, 0, 0, 5, 0, 0, 2, 0},
+                            {2, 6, 0, 0, 7, 0, 1, 2, 0}, {1, 0, 2, 1, 5, 6, 0, 2, 3}, {1, 0, 2, 0, 7, 8, 0, 2, 2},
+                            {1, 0, 0, 0, 7, 7, 1, 4, 0}, {2, 6, 0, 0, 5, 5, 1, 2, 0}};
+  for (int m = 0; m < 8; m++) k |= (uint32_t)(t[m].*f) << (4 * m);
+  return k;
 }
 
-// Every read below has a per-lane width (0 where the mode lacks the field)
-// rather than a per-lane branch, so a wave of mixed modes runs one path.
-BCN_HD void decode_bc7(const uint32_t in[4], uint32_t px[16]) {
-  uint32_t first = in[0] & 0xFFu;
-  if (first == 0) {
-#pragma unroll
-    for (int k = 0; k < 16; k++) px[k] = 0;  // invalid mode: transparent black
-    return;
-  }
-  int mode = __builtin_ctz(first);
-  const Bc7ModeInfo mi = kBc7Modes[mode];
+// Per-lane subset data of the BC7 decode: for subsets 0..2, c = 256 a + 128
+// and d = 4 (b - a) for endpoints a, b, in 16-bit lanes (R | G << 16 and
+// B | A << 16).  Texel k of subset s is then c + w d modulo 2^16 per lane,
+// which is 4 ((64 - w) a + w b + 32) < 2^16: the interpolated channel
+// (block.rs:1256-1258) is the lane's high byte.
+struct alignas(16) Bc7SubsetData {
+  uint32_t crg, cba, drg, dba;
+};
+
+// decode_block_bc7 (block.rs:801-1252) for one block, every mode in one
+// branch-free path.  `sub` holds this lane's three subset records at stride
+// `stride` (LDS rows on the GPU), read back by subset id per texel.
+//   * The mode's field widths come from nibble tables; a field the mode does
+//     not have is a read of width 0.
+//   * The mode bits, partition, rotation and index selector are one read; the
+//     six endpoints of a channel (ne * width <= 30 bits) are one read split
+//     by v_bfe_u32; so are the alpha endpoints and the p-bits.
+//   * Index set 1 is read from the two low words of what remains; index set 2
+//     (modes 4 and 5) from the top of the block, where it ends.  Anchor texels
+//     read one bit less, from a per-block anchor mask.
+//   * Weights are round(64 i / (2^bits - 1)) = (i * (4095 / m) + 32) >> 6 in
+//     16-bit lanes: the colour weight in the low lane, the alpha weight in the
+//     high lane (set 1 in both without a set 2; the index selector swaps the
+//     sets).  R and G take the low lane, B the low and A the high lane.
+//   * The rotation (block.rs:1228-1240) exchanges a colour channel and alpha
+//     after interpolation: it only changes the byte selector that packs the
+//     texel.
+BCN_HD void decode_bc7(const uint32_t in[4], uint32_t px[16], Bc7SubsetData* sub, int stride) {
+  // A first byte of 0 is no mode: the block decodes as transparent black.
+  // It takes mode 7's layout here and a byte selector of zeros at the end,
+  // so every lane runs the same path.
+  const uint32_t first = in[0] & 0xFFu;
+  const uint32_t mode = (uint32_t)__builtin_ctz(first | 0x80u), m4 = 4u * mode;
+  constexpr uint32_t kNs = bc7_nibbles(&Bc7ModeInfo::ns), kPb = bc7_nibbles(&Bc7ModeInfo::pb),
+                     kRb = bc7_nibbles(&Bc7ModeInfo::rb), kIsb = bc7_nibbles(&Bc7ModeInfo::isb),
+                     kCb = bc7_nibbles(&Bc7ModeInfo::cb), kAb = bc7_nibbles(&Bc7ModeInfo::ab),
+                     kPk = bc7_nibbles(&Bc7ModeInfo::pbit), kIb = bc7_nibbles(&Bc7ModeInfo::ib),
+                     kIb2 = bc7_nibbles(&Bc7ModeInfo::ib2);
+  const uint32_t ns = bfe_u32(kNs, m4, 4), pb = bfe_u32(kPb, m4, 4), rb = bfe_u32(kRb, m4, 4),
+                 isbits = bfe_u32(kIsb, m4, 4), cb = bfe_u32(kCb, m4, 4), ab = bfe_u32(kAb, m4, 4),
+                 pk = bfe_u32(kPk, m4, 4), ib = bfe_u32(kIb, m4, 4), ib2 = bfe_u32(kIb2, m4, 4);
   BitReader br(in);
-  br.skip((uint32_t)mode + 1);
-  int part = (int)br.read(mi.pb);
-  int rotation = (int)br.read(mi.rb);
-  int isb = (int)br.read(mi.isb);
-  const int ne = mi.ns * 2;
-  // Endpoint arrays are indexed with compile-time indices only (unrolled
-  // loops and selects) so they stay in registers.
+  const uint32_t h0 = mode + 1, h1 = h0 + pb, h2 = h1 + rb;
+  const uint32_t hdr = br.read(h2 + isbits);
+  const uint32_t part = bfe_u32(hdr, h0, pb), rotation = bfe_u32(hdr, h1, rb), isb = bfe_u32(hdr, h2, isbits);
+  const uint32_t ne = 2 * ns;
+  // Endpoint fields: per channel one read of ne * width bits (<= 30).  An
+  // endpoint the mode lacks reads above the chunk and gets 0.
   int ep[6][4];
 #pragma unroll
-  for (int c = 0; c < 3; c++)
+  for (int c = 0; c < 4; c++) {
+    const uint32_t w = c < 3 ? cb : ab;
+    const uint32_t chunk = br.read(ne * w);
 #pragma unroll
-    for (int e = 0; e < 6; e++) ep[e][c] = (int)br.read(e < ne ? mi.cb : 0u);
-#pragma unroll
-  for (int e = 0; e < 6; e++) ep[e][3] = (int)br.read(e < ne ? mi.ab : 0u);
-  // P-bits: one per endpoint (kind 1) or one per subset shared by its two
-  // endpoints' colour channels (kind 2); at most one of the two loops reads.
-  int pb[6];
-#pragma unroll
-  for (int e = 0; e < 6; e++) pb[e] = (int)br.read(mi.pbit == 1 && e < ne ? 1u : 0u);
-#pragma unroll
-  for (int sb = 0; sb < 2; sb++) {
-    int b = (int)br.read(mi.pbit == 2 ? 1u : 0u);
-    pb[2 * sb] |= b;
-    pb[2 * sb + 1] |= b;
+    for (int e = 0; e < 6; e++) ep[e][c] = (int)bfe_u32(chunk, (uint32_t)e * w, w);
   }
-  const int has_p = mi.pbit != 0, alpha_p = mi.pbit == 1;
-  const int cbits = mi.cb + has_p, abits = mi.ab + alpha_p;
-  // Expand, apply the rotation to the endpoints (output channel rotation-1
-  // takes the alpha endpoints and alpha the channel's, block.rs:1228-1240),
-  // and pack each subset's endpoints as (R | G << 16), (B | A << 16).
-  uint32_t lrg[3], lba[3], hrg[3], hba[3];
+  // P-bits: ne of them (kind 1, one per endpoint) or 2 (kind 2, one per
+  // subset shared by its two endpoints, spread here to one per endpoint),
+  // else none.
+  const uint32_t praw = br.read(pk == 1 ? ne : (pk == 2 ? 2u : 0u));
+  const uint32_t pch = pk == 2 ? (praw & 1u) * 3u + (praw & 2u) * 6u : praw;
+  const uint32_t has_p = pk != 0 ? 1u : 0u, alpha_p = pk == 1 ? 1u : 0u;
+  const uint32_t cbits = cb + has_p, abits = ab + alpha_p;
+  // Each endpoint as two words of 16-bit lanes, (R | G << 16) and
+  // (B | A << 16).  A field v of w bits with p-bit p (t = w + 1 bits) or
+  // without (t = w) is placed at the top of 8 bits by one multiply-add,
+  //   (2 v + p) << (8 - t) = v * 2^(8 - w) + p * 2^(7 - w),
+  // and expand_bits (block.rs:1159-1163) is then x | (x >> t).  A mode
+  // without alpha decodes alpha 255 (expand of 0 ORed with 255).
+  const uint32_t pc = has_p ? 1u << (7 - cb) : 0u, pa = alpha_p ? 1u << (7 - ab) : 0u;
+  const uint32_t mul_rg = (1u << (8 - cb)) * 0x10001u, mul_ba = (1u << (8 - cb)) | ((1u << (8 - ab)) << 16);
+  const uint32_t k_rg = pc * 0x10001u, k_ba = pc | (pa << 16);
+  const uint32_t bt_rg = cbits * 0x10001u, bt_ba = cbits | ((ab ? abits : 8u) << 16);
+  const uint32_t alpha_or = ab ? 0u : 0x00FF0000u;
 #pragma unroll
   for (int sb = 0; sb < 3; sb++) {
-    int v[2][4];
+    uint32_t rg[2], ba[2];
 #pragma unroll
     for (int h = 0; h < 2; h++) {
       const int e = 2 * sb + h;
-      int c[4];
-#pragma unroll
-      for (int ch = 0; ch < 3; ch++) c[ch] = expand_bits(has_p ? (ep[e][ch] << 1) | pb[e] : ep[e][ch], cbits);
-      c[3] = mi.ab ? expand_bits(alpha_p ? (ep[e][3] << 1) | pb[e] : ep[e][3], abits) : 255;
-      v[h][0] = rotation == 1 ? c[3] : c[0];
-      v[h][1] = rotation == 2 ? c[3] : c[1];
-      v[h][2] = rotation == 3 ? c[3] : c[2];
-      v[h][3] = rotation == 1 ? c[0] : (rotation == 2 ? c[1] : (rotation == 3 ? c[2] : c[3]));
+      const uint32_t p = bfe_u32(pch, (uint32_t)e, 1);
+      const uint32_t x = pk_mad16((uint32_t)ep[e][0] | ((uint32_t)ep[e][1] << 16), mul_rg, p * k_rg);
+      const uint32_t y = pk_mad16((uint32_t)ep[e][2] | ((uint32_t)ep[e][3] << 16), mul_ba, p * k_ba);
+      rg[h] = x | pk_shr16(x, bt_rg);
+      ba[h] = (y | pk_shr16(y, bt_ba)) | alpha_or;
     }
-    lrg[sb] = (uint32_t)v[0][0] | ((uint32_t)v[0][1] << 16);
-    lba[sb] = (uint32_t)v[0][2] | ((uint32_t)v[0][3] << 16);
-    hrg[sb] = (uint32_t)v[1][0] | ((uint32_t)v[1][1] << 16);
-    hba[sb] = (uint32_t)v[1][2] | ((uint32_t)v[1][3] << 16);
+    Bc7SubsetData d;
+    d.crg = pk_mad16(rg[0], 0x01000100u, 0x00800080u);
+    d.cba = pk_mad16(ba[0], 0x01000100u, 0x00800080u);
+    d.drg = pk_shl16(pk_sub16(rg[1], rg[0]), 0x00020002u);
+    d.dba = pk_shl16(pk_sub16(ba[1], ba[0]), 0x00020002u);
+    sub[sb * stride] = d;
   }
-  // Half-words interpolated with the alpha-set weight: the output channel
-  // that holds the alpha endpoints.
-  const uint32_t mrg = (rotation == 1 ? 0x0000FFFFu : 0u) | (rotation == 2 ? 0xFFFF0000u : 0u);
-  const uint32_t mba = (rotation == 3 ? 0x0000FFFFu : 0u) | (rotation == 0 ? 0xFFFF0000u : 0u);
-  uint32_t pattern = mi.ns == 1 ? 0u : kPart.subset[mi.ns == 3 ? 64 + part : part];
-  int a1 = mi.ns >= 2 ? kPart.anchor1[mi.ns == 3 ? 64 + part : part] : -1;
-  int a2 = mi.ns == 3 ? kPart.anchor2[64 + part] : -1;
-  const int k1 = bptc_weight_mul(mi.ib), k2 = bptc_weight_mul(mi.ib2);
-  int w1[16];
+  // Weight lanes: perm_b32(x2, x1, ipsel) places the set 1 and set 2 indices
+  // x1, x2 in the colour (low) and alpha (high) lanes, and kk the matching
+  // 4095 / m multipliers.
+  const uint32_t m1 = (uint32_t)bptc_weight_mul((int)ib), m2 = (uint32_t)bptc_weight_mul((int)ib2);
+  const uint32_t ipsel = ib2 == 0 ? 0x0C000C00u : (isb ? 0x0C000C04u : 0x0C040C00u);
+  const uint32_t kk = ib2 == 0 ? m1 * 0x10001u : (isb ? m2 | (m1 << 16) : m1 | (m2 << 16));
+  // Texel bytes: the high byte of each lane, R, G from the (R, G) word
+  // (bytes 1, 3) and B, A from the (B, A) word (bytes 5, 7), with the
+  // rotated channel and alpha exchanged.
+  const uint32_t out_sel = first == 0 ? 0x0C0C0C0Cu
+                         : (rotation == 1 ? 0x01050307u
+                            : (rotation == 2 ? 0x03050701u : (rotation == 3 ? 0x05070301u : 0x07050301u)));
+  const uint32_t pidx = ns == 3 ? 64u + part : part;
+  const uint32_t pattern = ns == 1 ? 0u : kPart.subset[pidx];
+  const uint32_t a1 = ns >= 2 ? kPart.anchor1[pidx] : 0u, a2 = ns == 3 ? kPart.anchor2[pidx] : 0u;
+  const uint32_t amask = 1u | (1u << a1) | (1u << a2);  // anchors of set 1
+  // Index set 1 sits at the bottom of what remains (at most 63 bits, in the
+  // two low words); index set 2 ends the block: T2 = 16 ib2 - 1 bits.
+  uint32_t i0 = br.w0, i1 = br.w1;
+  const uint32_t t2 = ib2 ? 16u * ib2 - 1u : 0u, sh = 64u - t2;  // 17, 33 or 64
+  uint32_t j0 = sh < 32 ? funnel_shr(in[3], in[2], sh) : (sh < 64 ? in[3] >> (sh - 32) : 0u);
+  uint32_t j1 = sh < 32 ? in[3] >> sh : 0u;
 #pragma unroll
   for (int k = 0; k < 16; k++) {
-    bool anchor = k == 0 || k == a1 || k == a2;
-    w1[k] = (mul24((int)br.read(anchor ? mi.ib - 1u : mi.ib), k1) + 32) >> 6;
+    const uint32_t n1 = ib - bfe_u32(amask, (uint32_t)k, 1);
+    const uint32_t n2 = k == 0 ? (ib2 ? ib2 - 1u : 0u) : ib2;
+    const uint32_t x1 = bfe_u32(i0, 0, n1), x2 = bfe_u32(j0, 0, n2);
+    i0 = funnel_shr(i1, i0, n1);
+    i1 >>= n1;
+    j0 = funnel_shr(j1, j0, n2);
+    j1 >>= n2;
+    // Both weights: i * (4095 / m) + 32 <= 4127 per lane, then >> 6.
+    const uint32_t wp = pk_mad_shr6(perm_b32(x2, x1, ipsel), kk, 0x00200020u);
+    const uint32_t s = bfe_u32(pattern, 2u * (uint32_t)k, 2);
+    const Bc7SubsetData d = sub[(int)s * stride];
+    const uint32_t rg = pk_mad16_lo(d.drg, wp, d.crg);
+    const uint32_t ba = pk_mad16(d.dba, wp, d.cba);
+    px[k] = perm_b32(ba, rg, out_sel);
   }
+}
+
+// The RGBA8 texels of one block of format FMT (bcn_variant BC1..BC5, BC7) as
+// decompress_blocks_as_rgba8 writes them (src/decode.rs:336-387): BC4 ->
+// (R, 0, 0, 0), BC5 -> (R, G, 0, 0).  `in` holds the block's 8 or 16 bytes;
+// `sub`/`stride` are BC7's subset records (decode_bc7).
+template <int FMT>
+BCN_HD void decode_rgba8_block(const uint32_t in[4], uint32_t px[16], Bc7SubsetData* sub, int stride) {
+  if (FMT == 0) {  // BC1
+    decode_color(in[0], in[1], false, px);
+  } else if (FMT == 1 || FMT == 2) {  // BC2, BC3: colour, then alpha in byte 3
+    decode_color(in[2], in[3], true, px);
+    if (FMT == 1) {
+      // decode_sharp_alpha_block (block.rs:200-217): 4-bit alpha x 17
 #pragma unroll
-  for (int k = 0; k < 16; k++) {
-    int s = (int)((pattern >> (2 * k)) & 3u);
-    // Second index set (modes 4 and 5), read after all of the first.
-    int i2 = (int)br.read(mi.ib2 == 0 ? 0u : (k == 0 ? mi.ib2 - 1u : mi.ib2));
-    int w2 = mi.ib2 == 0 ? w1[k] : (mul24(i2, k2) + 32) >> 6;
-    uint32_t wc = (uint32_t)(isb ? w2 : w1[k]), wa = (uint32_t)(isb ? w1[k] : w2);
-    wc |= wc << 16;
-    wa |= wa << 16;
-    uint32_t rg = interp2(s == 0 ? lrg[0] : (s == 1 ? lrg[1] : lrg[2]),
-                          s == 0 ? hrg[0] : (s == 1 ? hrg[1] : hrg[2]), (wc & ~mrg) | (wa & mrg));
-    uint32_t ba = interp2(s == 0 ? lba[0] : (s == 1 ? lba[1] : lba[2]),
-                          s == 0 ? hba[0] : (s == 1 ? hba[1] : hba[2]), (wc & ~mba) | (wa & mba));
-    px[k] = (rg & 0xFFu) | ((rg >> 8) & 0xFF00u) | ((ba & 0xFFu) << 16) | ((ba >> 16) << 24);
+      for (int k = 0; k < 16; k++) {
+        uint32_t a = ((k < 8 ? in[0] : in[1]) >> (4 * (k & 7))) & 15u;
+        px[k] = (px[k] & 0x00FFFFFFu) | ((a * 17u) << 24);
+      }
+    } else {
+      uint32_t a4[4];
+      decode_channel4(in[0], in[1], a4);
+#pragma unroll
+      for (int k = 0; k < 16; k++) px[k] = perm_b32(a4[k >> 2], px[k], 0x00020100u | ((4u + (k & 3)) << 24));
+    }
+  } else if (FMT == 3) {  // BC4 (decode.rs:97-118)
+    uint32_t r4[4];
+    decode_channel4(in[0], in[1], r4);
+#pragma unroll
+    for (int k = 0; k < 16; k++) px[k] = bfe_u32(r4[k >> 2], 8u * (k & 3), 8);
+  } else if (FMT == 4) {  // BC5
+    uint32_t r4[4], g4[4];
+    decode_channel4(in[0], in[1], r4);
+    decode_channel4(in[2], in[3], g4);
+#pragma unroll
+    for (int k = 0; k < 16; k++) px[k] = perm_b32(g4[k >> 2], r4[k >> 2], 0x0C0C0400u | ((k & 3) * 0x101u));
+  } else {  // BC7
+    decode_bc7(in, px, sub, stride);
   }
 }
 
 // ---- BC6H (block.rs:262-769) --------------------------------------------------
 
-BCN_HD int extend_sign(int v, int bits) { return (int)((uint32_t)v << (32 - bits)) >> (32 - bits); }
+// The low `bits` bits of v (1..32) as a signed value (v_bfe_i32 on the GPU).
+BCN_HD int extend_sign(int v, int bits) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_sbfe(v, 0, bits);
+#else
+  return (int)((uint32_t)v << (32 - bits)) >> (32 - bits);
+#endif
+}
 
 // unquantize (block.rs:1369-1389), written as selects.
 template <bool SIGNED>
@@ -268,18 +355,16 @@ BCN_HD int bc6h_unquantize(int v, int bits) {
 // value whose magnitude scales to 0 gives +0.
 template <bool SIGNED>
 BCN_HD uint16_t bc6h_finish(int v) {
-  if (!SIGNED) return (uint16_t)((v * 31) >> 6);
-  uint32_t mag = (uint32_t)(((v < 0 ? -v : v) * 31) >> 5);
+  // |v| < 2^17: the products are 24-bit multiplies.
+  if (!SIGNED) return (uint16_t)(mul24i(v, 31) >> 6);
+  uint32_t mag = (uint32_t)(mul24i(v < 0 ? -v : v, 31) >> 5);
   return (uint16_t)((v < 0 && mag != 0) ? (0x8000u | mag) : mag);
 }
 
-// Base-endpoint bits per mode (the decoder's W precision).
-BCN_TABLE uint8_t kBc6hBaseBits[14] = {10, 7, 11, 11, 11, 9, 8, 8, 8, 6, 10, 11, 12, 16};
-
-// The low n bits of v in reverse order (n in 1..31).
+// The low n bits of v in reverse order (n in 0..31; 0 gives 0).
 BCN_HD uint32_t reverse_low_bits(uint32_t v, uint32_t n) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  return __builtin_bitreverse32(v) >> (32u - n);
+  return n ? __builtin_bitreverse32(v) >> (32u - n) : 0u;
 #else
   uint32_t r = 0;
   for (uint32_t k = 0; k < n; k++) r |= ((v >> k) & 1u) << (n - 1 - k);
@@ -287,47 +372,167 @@ BCN_HD uint32_t reverse_low_bits(uint32_t v, uint32_t n) {
 #endif
 }
 
-// RGB half-float bit patterns of the 16 texels: out[k*3 + c].  As in
-// decode_bc7, per-lane differences (mode, field widths) are read widths and
-// selects, not branches.
+// Decoder form of kBc6hFields: width | destination << 5 | reversed << 15,
+// where the destination is the field's first bit in the 192-bit value that
+// holds the 12 endpoint components as 16-bit slots (slot = endpoint * 3 +
+// channel, bit = slot * 16 + first_bit).  Entries past a mode's count are 0:
+// a read of no bits ORed into bit 0.  Row 14 is all 0, the row of the
+// reserved mode prefixes: no fields, so every component is 0.
+constexpr int kBc6hDecRows = 15;
+struct Bc6hDecFields {
+  uint16_t f[kBc6hDecRows][23];
+};
+constexpr Bc6hDecFields make_bc6h_dec_fields() {
+  Bc6hDecFields t{};
+  for (int m = 0; m < 14; m++)
+    for (int i = 0; i < 23; i++) {
+      const uint32_t e = kBc6hFieldSrc.f[m][i];
+      const uint32_t c = e & 3u, ep = (e >> 2) & 3u, first = (e >> 4) & 31u, n = (e >> 9) & 63u;
+      const uint32_t dst = (ep * 3 + c) * 16 + first;
+      t.f[m][i] = (uint16_t)(n | (dst << 5) | ((e >> 15) << 15));
+    }
+  return t;
+}
+BCN_TABLE Bc6hDecFields kBc6hDec = make_bc6h_dec_fields();
+
+// Every field fits its 16-bit slot, and the packed form round-trips.
+constexpr bool bc6h_dec_fields_hold() {
+  const Bc6hDecFields t = make_bc6h_dec_fields();
+  for (int m = 0; m < 14; m++)
+    for (int i = 0; i < 23; i++) {
+      const uint32_t e = t.f[m][i], n = e & 31u, dst = (e >> 5) & 0xFFu;
+      const uint32_t src = kBc6hFieldSrc.f[m][i];
+      if (n != ((src >> 9) & 63u) || (dst & 15u) + n > 16) return false;
+      if (dst >> 4 != ((src >> 2) & 3u) * 3 + (src & 3u) || (e >> 15) != (src >> 15u)) return false;
+    }
+  return true;
+}
+static_assert(bc6h_dec_fields_hold(), "BC6H decoder field table");
+
+// Mode prefix (5 bits, or 2 for prefixes 0 and 1) -> mode index + 1 as a
+// nibble table over the 32 five-bit values; 0 marks the reserved prefixes.
+constexpr uint32_t bc6h_mode_nibbles(int word) {
+  const uint8_t prefix[14] = {0, 1, 2, 6, 10, 14, 18, 22, 26, 30, 3, 7, 11, 15};
+  uint32_t k = 0;
+  for (int v = 8 * word; v < 8 * word + 8; v++) {
+    int m = -1;
+    for (int i = 0; i < 14; i++)
+      if ((i < 2 ? (v & 3) : v) == prefix[i] && (i < 2 ? (v & 3) < 2 : (v & 3) >= 2)) m = i;
+    k |= (uint32_t)(m + 1) << (4 * (v - 8 * word));
+  }
+  return k;
+}
+
+// Base bits (which = 0) or the delta bits of R, G, B (which = 1..3) of the 14
+// modes, minus one, as nibbles (mode m at bits 4m of the low word for m < 8,
+// 4(m-8) of the high word otherwise), from kBc6hPrecSrc.
+constexpr uint64_t bc6h_prec_nibbles(int which) {
+  uint64_t k = 0;
+  for (int m = 0; m < 14; m++) {
+    const int v = which == 0 ? kBc6hPrecSrc.base[m] : kBc6hPrecSrc.delta[which - 1][m];
+    k |= (uint64_t)(v - 1) << (4 * (m < 8 ? m : m - 8) + (m < 8 ? 0 : 32));
+  }
+  return k;
+}
+
+// Per-lane data of the BC6H decode held in LDS on the GPU: words 0..5 the
+// 12 endpoint components as 16-bit slots (ORed field by field), words 6..17
+// the two subsets' interpolation records (c = 64 lo + 32, d = hi - lo per
+// channel).  Rows are interleaved by lane at `stride` words.
+struct Bc6hScratch {
+  uint32_t* row;
+  int stride;
+  BCN_HD uint32_t& at(int w) const { return row[w * stride]; }
+};
+
+// Whether any lane of the wave has `p` (the lane's own value on the host).
+BCN_HD bool wave_any(bool p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __ballot(p) != 0;
+#else
+  return p;
+#endif
+}
+
+// BC6H index state after the block's header: the remaining index bits and
+// what the texels need to read them (decode_bc6h_texel reads them in order).
+struct Bc6hIdx {
+  uint32_t j0, j1;   // index bits not yet read, lowest first
+  uint32_t amask;    // anchor texels (read one bit less)
+  uint32_t pattern;  // 2-bit subset ids of the 16 texels
+  uint32_t ib;       // index bits
+  int kw;            // weight multiplier: w = (i * kw + 32) >> 6
+};
+
+// decode_block_bc6h (block.rs:262-769), the header: mode, endpoint fields,
+// transforms and unquantize, leaving each subset's interpolation records in
+// the lane's scratch row (words 6..17) and returning the index state.  Every
+// mode runs one branch-free path: field widths, destinations and precisions
+// come from tables indexed by the lane's mode, and a field the mode lacks is
+// a read of width 0.
+//   * Each endpoint field is read and ORed into its 16-bit slot in the lane's
+//     scratch row (an LDS OR on the GPU), rather than merged through selects
+//     over all 12 slots.  `fields` is kBc6hDec (a copy in LDS on the GPU).
+//   * A reserved mode prefix takes the field row of no fields: its
+//     components are all 0 and so is every texel (block.rs:282-290).
+//   * The indices end the block, so they are read from its top 64 bits;
+//     anchor texels read one bit less, from an anchor mask.
+//   * Interpolation lo + ((w (hi - lo) + 32) >> 6) is (c + w d) >> 6 with
+//     c = 64 lo + 32 (64 lo is a multiple of 64): one 24-bit multiply-add.
 template <bool SIGNED>
-BCN_HD void decode_bc6h_t(const uint32_t in[4], uint16_t out[48]) {
+BCN_HD Bc6hIdx decode_bc6h_header(const uint32_t in[4], Bc6hScratch sc, const uint16_t* fields) {
   BitReader br(in);
   uint32_t mb = br.read(2);
   mb |= br.read(mb > 1 ? 3u : 0u) << 2;
-  int m = -1;
+  constexpr uint32_t kM0 = bc6h_mode_nibbles(0), kM1 = bc6h_mode_nibbles(1), kM2 = bc6h_mode_nibbles(2),
+                     kM3 = bc6h_mode_nibbles(3);
+  const uint32_t mw = mb >> 3;
+  const uint32_t mk = mw == 0 ? kM0 : (mw == 1 ? kM1 : (mw == 2 ? kM2 : kM3));
+  const int m1 = (int)bfe_u32(mk, 4u * (mb & 7u), 4);
+  const int m = m1 ? m1 - 1 : kBc6hDecRows - 1;
 #pragma unroll
-  for (int i = 0; i < 14; i++) m = (kBc6hPrefix[i] == mb) ? i : m;
-  if (m < 0) {
-    for (int i = 0; i < 48; i++) out[i] = 0;  // reserved mode: RGB zero
-    return;
-  }
-  // Field value of (endpoint e, channel c) goes to slot e*3 + c; slots are
-  // kept as 16-bit halves of six words (every field value is below 2^16
-  // before sign extension) and merged through selects, so they stay in
-  // registers although the slot differs by lane.  Every mode runs the
-  // longest field list: the table's entries past a mode's count are 0, a
-  // read of 0 bits into slot 0.
-  uint32_t f2[6] = {0, 0, 0, 0, 0, 0};
+  for (int w = 0; w < 6; w++) sc.at(w) = 0;
+  const uint16_t* ft = fields + m * 23;
+  // Every field is merged as read; the reversed ones are put in order below.
 #pragma unroll
   for (int i = 0; i < 23; i++) {
-    uint32_t t = kBc6hFields[m][i];
-    uint32_t c = t & 3u, e = (t >> 2) & 3u, first = (t >> 4) & 31u, n = (t >> 9) & 63u;
-    uint32_t raw = br.read(n);
-    // A reversed field stores its bits from first+n-1 down to first.
-    uint32_t v = ((t >> 15) ? reverse_low_bits(raw, n) : raw) << first;
-    uint32_t slot = e * 3 + c;
-    v <<= (slot & 1u) * 16;
-#pragma unroll
-    for (uint32_t w = 0; w < 6; w++) f2[w] |= w == (slot >> 1) ? v : 0u;
+    const uint32_t t = ft[i];
+    const uint32_t n = t & 31u, dst = (t >> 5) & 0xFFu;
+    const uint32_t v = br.read(n);
+#if defined(__HIP_DEVICE_COMPILE__)
+    __hip_atomic_fetch_or(&sc.at((int)(dst >> 5)), v << (dst & 31u), __ATOMIC_RELAXED,
+                          __HIP_MEMORY_SCOPE_WORKGROUP);
+#else
+    sc.at((int)(dst >> 5)) |= v << (dst & 31u);
+#endif
   }
   int f[12];
 #pragma unroll
-  for (int i = 0; i < 12; i++) f[i] = (int)((f2[i >> 1] >> (16 * (i & 1))) & 0xFFFFu);
+  for (int w = 0; w < 6; w++) {
+    const uint32_t x = sc.at(w);
+    f[2 * w] = (int)(x & 0xFFFFu);
+    f[2 * w + 1] = (int)(x >> 16);
+  }
+  // The reversed fields are the top 2 (mode index 12) or 6 (13) bits of each
+  // base component, from bit 10, stored last bit first (block.rs:656-676):
+  // reverse them in place.
+  const uint32_t nrev = m == 12 ? 2u : (m == 13 ? 6u : 0u);
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const uint32_t x = (uint32_t)f[c];
+    f[c] = nrev ? (int)((x & 0x3FFu) | (reverse_low_bits(x >> 10, nrev) << 10)) : f[c];
+  }
   const bool two = m < 10;
-  int part = (int)br.read(two ? 5u : 0u);
+  const uint32_t part = br.read(two ? 5u : 0u);
   const int ne = two ? 4 : 2;
-  const int b0 = kBc6hBaseBits[m];
+  // Base bits (6..16) and the delta bits of R, G, B (kBc6hPrec), minus one,
+  // as nibbles: modes 0..7 in the low word (the reserved row reads 0).
+  constexpr uint64_t kBase = bc6h_prec_nibbles(0), kDR = bc6h_prec_nibbles(1), kDG = bc6h_prec_nibbles(2),
+                     kDB = bc6h_prec_nibbles(3);
+  const uint32_t mo = 4u * (uint32_t)(m & 7), mlo = m < 8;
+  auto nib = [&](uint64_t k) { return (int)bfe_u32(mlo ? (uint32_t)k : (uint32_t)(k >> 32), mo, 4) + 1; };
+  const int b0 = nib(kBase);
+  const int db[3] = {nib(kDR), nib(kDG), nib(kDB)};
   const bool transformed = m != 9 && m != 10;
   if (SIGNED)
 #pragma unroll
@@ -338,9 +543,9 @@ BCN_HD void decode_bc6h_t(const uint32_t in[4], uint16_t out[48]) {
     for (int c = 0; c < 3; c++) {
       int v = f[e * 3 + c];
       if (SIGNED) {
-        v = extend_sign(v, kBc6hDeltaBits[c][m]);
+        v = extend_sign(v, db[c]);
       } else {
-        v = transformed ? extend_sign(v, kBc6hDeltaBits[c][m]) : v;
+        v = transformed ? extend_sign(v, db[c]) : v;
       }
       int t = (v + f[c]) & ((1 << b0) - 1);
       v = transformed ? (SIGNED ? extend_sign(t, b0) : t) : v;
@@ -348,64 +553,111 @@ BCN_HD void decode_bc6h_t(const uint32_t in[4], uint16_t out[48]) {
     }
 #pragma unroll
   for (int i = 0; i < 12; i++) f[i] = bc6h_unquantize<SIGNED>(f[i], b0);
-  uint32_t pattern = two ? kPart.subset[part] : 0u;
-  int a1 = two ? kPart.anchor1[part] : -1;
-  const int ib = two ? 3 : 4;
-  const int kw = bptc_weight_mul(ib);
+  // Subset records: c = 64 lo + 32 and d = hi - lo per channel (|d| < 2^17,
+  // 64 |d| + c < 2^23: one v_mad_i32_i24).
 #pragma unroll
-  for (int k = 0; k < 16; k++) {
-    bool anchor = k == 0 || k == a1;
-    int w = (mul24i((int)br.read(anchor ? ib - 1 : ib), kw) + 32) >> 6;
-    int s = (int)((pattern >> (2 * k)) & 1u);
+  for (int sb = 0; sb < 2; sb++)
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      int lo = s ? f[6 + c] : f[c], hi = s ? f[9 + c] : f[3 + c];
-      // (lo*(64-w) + hi*w + 32) >> 6 == lo + ((w*(hi-lo) + 32) >> 6): 64*lo is
-      // a multiple of 64 and >> floors; |w*(hi-lo)| <= 64 * 65535 < 2^23.
-      out[k * 3 + c] = bc6h_finish<SIGNED>(lo + ((mul24i(w, hi - lo) + 32) >> 6));
+      const int lo = f[6 * sb + c], hi = f[6 * sb + 3 + c];
+      sc.at(6 + 6 * sb + c) = (uint32_t)(64 * lo + 32);
+      sc.at(6 + 6 * sb + 3 + c) = (uint32_t)(hi - lo);
     }
+  Bc6hIdx st;
+  st.pattern = two ? kPart.subset[part] : 0u;
+  const uint32_t a1 = two ? kPart.anchor1[part] : 0u;
+  st.amask = 1u | (1u << a1);
+  st.ib = two ? 3u : 4u;
+  st.kw = two ? 585 : 273;
+  // The indices: the block's last 16 ib - (1 + two) bits (46 or 63).
+  const uint32_t sh = two ? 18u : 1u;
+  st.j0 = funnel_shr(in[3], in[2], sh);
+  st.j1 = in[3] >> sh;
+  return st;
+}
+
+// Texel k's RGB half bit patterns; texels are read in order 0..15.
+template <bool SIGNED>
+BCN_HD void decode_bc6h_texel(Bc6hIdx& st, int k, Bc6hScratch sc, uint16_t rgb[3]) {
+  const uint32_t n = st.ib - bfe_u32(st.amask, (uint32_t)k, 1);
+  const uint32_t x = bfe_u32(st.j0, 0, n);
+  st.j0 = funnel_shr(st.j1, st.j0, n);
+  st.j1 >>= n;
+  const int w = (mul24((int)x, st.kw) + 32) >> 6;
+  const int s = (int)bfe_u32(st.pattern, 2u * (uint32_t)k, 1);
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const int cc = (int)sc.at(6 + 6 * s + c), dd = (int)sc.at(6 + 6 * s + 3 + c);
+    rgb[c] = bc6h_finish<SIGNED>((mul24i(w, dd) + cc) >> 6);
   }
 }
 
+// The whole block: out[k*3 + c] for texel k, channel c.
+template <bool SIGNED>
+BCN_HD void decode_bc6h_t(const uint32_t in[4], uint16_t out[48], Bc6hScratch sc,
+                          const uint16_t* fields) {
+  Bc6hIdx st = decode_bc6h_header<SIGNED>(in, sc, fields);
+#pragma unroll
+  for (int k = 0; k < 16; k++) decode_bc6h_texel<SIGNED>(st, k, sc, out + 3 * k);
+}
+
+// Host form with private scratch (tests and the CPU kernel-check build).
 BCN_HD void decode_bc6h(const uint32_t in[4], bool is_signed, uint16_t out[48]) {
+  uint32_t row[18];
+  const Bc6hScratch sc{row, 1};
+  const uint16_t* ft = &kBc6hDec.f[0][0];
   if (is_signed)
-    decode_bc6h_t<true>(in, out);
+    decode_bc6h_t<true>(in, out, sc, ft);
   else
-    decode_bc6h_t<false>(in, out);
+    decode_bc6h_t<false>(in, out, sc, ft);
+}
+
+// ---- sRGB8 codes of BC6H texels (linear_to_srgb, src/decode.rs:151-159) -----
+//
+// Over the non-negative half floats the code is a step function of the bit
+// pattern h (they order like their values): code(h) = #{k >= 1 : h >= t[k]}
+// for the 255 thresholds t[k], the smallest pattern whose code is >= k.
+// Within any bucket of 8 consecutive patterns the code rises at most once
+// (the steepest stretch of the curve rises by ~0.11 codes per half ulp), so
+// one 16-bit entry per bucket holds it:
+//   entry = 256 * base + 32 * (8 - off),
+// with base the code of the bucket's first pattern and off in 1..8 the
+// position of the rise (8: none).  For j = h & 7,
+//   entry + 32 j = 256 * base + 32 * (8 - off + j),  8 - off + j in [1, 14],
+// so its high byte is base + (j >= off), the code of h.
+constexpr uint32_t kSrgb8Buckets = 0x7C00u >> 3;  // every finite non-negative half
+
+// A 16-bit word whose high byte is the sRGB8 code of the non-negative
+// finite half h (h <= 0x7BFF).
+BCN_HD uint32_t srgb8_word(uint32_t h, const uint16_t* buckets) {
+  return (uint32_t)buckets[h >> 3] + ((h & 7u) << 5);
+}
+
+// Row and column of block `idx` in an image `bw` blocks wide, without a
+// division: m = floor((2^32 - 1) / bw) gives q = mulhi(idx, m) in
+// {idx / bw - 1, idx / bw} for idx < 2^31 (m bw > 2^32 - 1 - bw), and one
+// compare of the remainder settles it.  bw and the row count below 2^24
+// (the remainder product is a 24-bit multiply).
+struct BlockRows {
+  uint32_t bw, m;
+};
+BCN_HD BlockRows block_rows(uint32_t bw) { return BlockRows{bw, 0xFFFFFFFFu / bw}; }
+BCN_HD void block_pos(BlockRows r, uint32_t idx, uint32_t& yy, uint32_t& xx) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const uint32_t q = __umulhi(idx, r.m);
+  const uint32_t rem = idx - __umul24(q, r.bw);
+#else
+  const uint32_t q = (uint32_t)(((uint64_t)idx * r.m) >> 32);
+  const uint32_t rem = idx - q * r.bw;
+#endif
+  const bool up = rem >= r.bw;
+  yy = q + (up ? 1u : 0u);
+  xx = up ? rem - r.bw : rem;
 }
 
 #if defined(__HIPCC__)
-// sRGB8 code thresholds of linear_to_srgb (src/decode.rs:151-159) over the
-// non-negative half floats: t[k] is the smallest half bit pattern whose code
-// is >= k (t[0] = 0).  Non-negative halves order like their bit patterns,
-// so code(h) = #{k >= 1 : h >= t[k]}.
-struct Srgb8Table {
-  uint16_t t[256];
-};
-
 __device__ __forceinline__ float half_to_float(uint16_t h) {
   return (float)__builtin_bit_cast(_Float16, h);
-}
-
-// sRGB8 code estimate of a non-negative half from the hardware log2/exp2;
-// within one code of linear_to_srgb on every input.
-__device__ __forceinline__ int srgb8_estimate(uint16_t h) {
-  float lin = half_to_float(h);
-  // Both branches are computed and selected: no per-lane branch.
-  float curve = 1.055f * __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(lin) * (1.0f / 2.4f)) - 0.055f;
-  float v = lin <= 0.0031308f ? lin * 12.92f : curve;
-  return (int)(fminf(fmaxf(v, 0.0f), 1.0f) * 255.0f + 0.5f);
-}
-
-// linear_to_srgb of an unsigned BC6H texel value, exactly: the estimate is
-// corrected by one comparison against each neighbouring threshold (both
-// thresholds are read and the correction is a select).
-__device__ __forceinline__ uint32_t srgb8_code(uint16_t h, const uint16_t* lt) {
-  int k = srgb8_estimate(h);
-  const uint32_t below = lt[k], above = lt[k < 255 ? k + 1 : 255];
-  // Bitwise, not short-circuit, conditions: both reads stay unconditional.
-  const int dn = (int)((k > 0) & (h < below)), up = (int)((k < 255) & (h >= above));
-  return (uint32_t)(k - dn + (up & (dn ^ 1)));
 }
 #endif
 

@@ -93,3 +93,14 @@ def bc6h_srgb8_thresholds():
     out = np.zeros(256, np.uint16)
     _capi.lib().bcn_bc6h_srgb8_thresholds(out.ctypes.data_as(ctypes.c_void_p))
     return out
+
+
+def bc6h_srgb8_buckets():
+    """The 3,968 bucket entries the BC6H -> RGBA8 kernel reads (numpy uint16):
+    the sRGB8 code of a half bit pattern h <= 0x7BFF is the high byte of
+    entry[h >> 3] + 32 * (h & 7)."""
+    import numpy as np
+
+    out = np.zeros(3968, np.uint16)
+    _capi.check(_capi.lib().bcn_bc6h_srgb8_buckets(out.ctypes.data_as(ctypes.c_void_p)))
+    return out

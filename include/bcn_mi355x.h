@@ -195,6 +195,14 @@ int bcn_decode_bc6h_rgba16f(const void* blocks, size_t blocks_size, uint32_t wid
  * table checks without a GPU. */
 void bcn_bc6h_srgb8_thresholds(uint16_t* out);
 
+/* The 3,968 bucket entries the BC6H -> RGBA8 decode reads per channel, built
+ * from those thresholds: for a half bit pattern h <= 0x7BFF, the high byte of
+ * out[h >> 3] + 32 * (h & 7) is its sRGB8 code.  Returns BCN_OK, or
+ * BCN_ERR_UNSUPPORTED if the thresholds rise twice within 8 patterns (the
+ * entries cannot express that; the decode then fails the same way).  For
+ * table checks without a GPU. */
+int bcn_bc6h_srgb8_buckets(uint16_t* out);
+
 /* ---- DDS container ------------------------------------------------------------------
  *
  * The reference CLI writes its blocks as a DDS file through the `ddsfile`

@@ -5,8 +5,9 @@
 // are counted per exponent field (bits 23..30).  Mode 0 tests the functions as
 // the library uses them; mode 1 their unguarded fast paths, to show where the
 // guards are needed.  The second check sweeps the BC6H -> RGBA8 decode's sRGB8
-// code (srgb8_estimate / srgb8_code in decode_block.hpp) over every
-// non-negative finite half; the third, rsqrt_cr against 1.0f / sqrtf(x).
+// code (srgb8_word in decode_block.hpp, from an LDS copy of the bucket table
+// as the kernel reads it) over every non-negative finite half; the third,
+// rsqrt_cr against 1.0f / sqrtf(x).
 // Test infrastructure only: the library does not link this.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -63,32 +64,32 @@ extern "C" int fastmath_check(uint32_t lo, uint32_t hi, int mode, unsigned long 
 }
 
 // ---- sRGB8 codes of the BC6H -> RGBA8 decode ---------------------------------
-// For every half bit pattern h in [0, n): est[h] = srgb8_estimate(h) and
-// code[h] = srgb8_code(h, table) (block_compression_amd/csrc/decode_block.hpp),
-// with `table` the library's thresholds (bcn_bc6h_srgb8_thresholds).
+// For every half bit pattern h in [0, n): code[h] = high byte of
+// srgb8_word(h, table) (block_compression_amd/csrc/decode_block.hpp), with
+// `table` the library's bucket entries (bcn_bc6h_srgb8_buckets) in LDS.
 
-__global__ void srgb8_kernel(uint32_t n, bcn::Srgb8Table tab, uint8_t* est, uint8_t* code) {
-  __shared__ uint16_t lt[256];
-  lt[threadIdx.x] = tab.t[threadIdx.x];
+__global__ void srgb8_kernel(uint32_t n, const uint16_t* buckets, uint8_t* code) {
+  __shared__ uint16_t lb[bcn::kSrgb8Buckets];
+  for (uint32_t i = threadIdx.x; i < bcn::kSrgb8Buckets; i += 256) lb[i] = buckets[i];
   __syncthreads();
   const uint32_t h = blockIdx.x * 256 + threadIdx.x;
   if (h >= n) return;
-  est[h] = (uint8_t)bcn::srgb8_estimate((uint16_t)h);
-  code[h] = (uint8_t)bcn::srgb8_code((uint16_t)h, lt);
+  code[h] = (uint8_t)(bcn::srgb8_word(h, lb) >> 8);
 }
 
-extern "C" int srgb8_check(const uint16_t* table, uint32_t n, uint8_t* est, uint8_t* code) {
+extern "C" int srgb8_check(const uint16_t* buckets, uint32_t n, uint8_t* code) {
   if (n > 0x7C00u) return -1;
-  bcn::Srgb8Table tab;
-  for (int k = 0; k < 256; k++) tab.t[k] = table[k];
   uint8_t* d = nullptr;
-  hipError_t e = hipMalloc(&d, 2 * 0x7C00);
+  hipError_t e = hipMalloc(&d, 0x7C00 + 2 * bcn::kSrgb8Buckets);
   if (e != hipSuccess) return (int)e;
-  hipLaunchKernelGGL(srgb8_kernel, dim3((n + 255) / 256), dim3(256), 0, 0, n, tab, d, d + 0x7C00);
-  e = hipGetLastError();
+  uint16_t* t = reinterpret_cast<uint16_t*>(d + 0x7C00);
+  e = hipMemcpy(t, buckets, 2 * bcn::kSrgb8Buckets, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(srgb8_kernel, dim3((n + 255) / 256), dim3(256), 0, 0, n, t, d);
+    e = hipGetLastError();
+  }
   if (e == hipSuccess) e = hipDeviceSynchronize();
-  if (e == hipSuccess) e = hipMemcpy(est, d, n, hipMemcpyDeviceToHost);
-  if (e == hipSuccess) e = hipMemcpy(code, d + 0x7C00, n, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(code, d, n, hipMemcpyDeviceToHost);
   (void)hipFree(d);
   return (int)e;
 }

@@ -150,26 +150,14 @@ extern "C" int kcheck_decode(int fmt, int is_signed, const uint8_t* blocks, size
       continue;
     }
     uint32_t px[16];
-    if (fmt == 0) bcn::decode_color(in[0], in[1], false, px);
-    else if (fmt == 1 || fmt == 2) {
-      bcn::decode_color(in[2], in[3], true, px);
-      uint8_t a[16];
-      if (fmt == 1)
-        for (int k = 0; k < 16; k++) a[k] = (uint8_t)((((k < 8 ? in[0] : in[1]) >> (4 * (k & 7))) & 15u) * 17u);
-      else
-        bcn::decode_channel(in[0], in[1], a);
-      for (int k = 0; k < 16; k++) px[k] = (px[k] & 0x00FFFFFFu) | ((uint32_t)a[k] << 24);
-    } else if (fmt == 3) {
-      uint8_t r[16];
-      bcn::decode_channel(in[0], in[1], r);
-      for (int k = 0; k < 16; k++) px[k] = r[k];
-    } else if (fmt == 4) {
-      uint8_t r[16], g[16];
-      bcn::decode_channel(in[0], in[1], r);
-      bcn::decode_channel(in[2], in[3], g);
-      for (int k = 0; k < 16; k++) px[k] = r[k] | ((uint32_t)g[k] << 8);
-    } else {
-      bcn::decode_bc7(in, px);
+    bcn::Bc7SubsetData sub[3];
+    switch (fmt) {
+      case 0: bcn::decode_rgba8_block<0>(in, px, sub, 1); break;
+      case 1: bcn::decode_rgba8_block<1>(in, px, sub, 1); break;
+      case 2: bcn::decode_rgba8_block<2>(in, px, sub, 1); break;
+      case 3: bcn::decode_rgba8_block<3>(in, px, sub, 1); break;
+      case 4: bcn::decode_rgba8_block<4>(in, px, sub, 1); break;
+      default: bcn::decode_rgba8_block<6>(in, px, sub, 1); break;
     }
     std::memcpy((uint32_t*)out + i * 16, px, sizeof(px));
   }
@@ -313,4 +301,10 @@ extern "C" long kcheck_quant8_mismatches() {
   bcn::quant8_consts(0, c1, rq);
   if (bcn::quant8_index(0, c1, rq) != 1) bad++;
   return bad;
+}
+
+// block_pos (decode_block.hpp) for n (bw, idx) pairs: rows[i], cols[i].
+extern "C" void kcheck_block_pos(const uint32_t* bw, const uint32_t* idx, size_t n, uint32_t* rows,
+                                 uint32_t* cols) {
+  for (size_t i = 0; i < n; i++) bcn::block_pos(bcn::block_rows(bw[i]), idx[i], rows[i], cols[i]);
 }

@@ -126,3 +126,12 @@ def bc7_refine(blocks: np.ndarray, target: np.ndarray, steps: int, lr: float, ch
     lib().kcheck_bc7_refine(_p(blocks), _p(target), n, steps, ctypes.c_float(lr), channels, reassign,
                             _p(out), _p(s0), _p(s1))
     return out, s0, s1
+
+
+def block_pos(bw: np.ndarray, idx: np.ndarray):
+    bw = np.ascontiguousarray(bw, np.uint32)
+    idx = np.ascontiguousarray(idx, np.uint32)
+    rows = np.zeros(len(bw), np.uint32)
+    cols = np.zeros(len(bw), np.uint32)
+    lib().kcheck_block_pos(_p(bw), _p(idx), ctypes.c_size_t(len(bw)), _p(rows), _p(cols))
+    return rows, cols

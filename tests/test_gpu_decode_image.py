@@ -3,7 +3,7 @@ _rgba32f, src/decode.rs:336-468) against the oracle.
 
 * the BC6H -> RGBA8 sRGB8 code equals the oracle's linear_to_srgb on every
   non-negative finite half (exhaustive, through the kernel's own device
-  function), and its hardware estimate is within one code everywhere;
+  function and an LDS copy of the library's bucket table);
 * BC6H -> RGBA8 images (random blocks and a GPU-encoded HDR image) equal the
   oracle's decompress_blocks_as_rgba8 byte for byte;
 * RGBA16F / RGBA32F with the reference layout equal the oracle element for
@@ -35,15 +35,13 @@ def test_srgb8_code_exhaustive_on_every_half():
 
     print("decode-image: sRGB8 code sweep", flush=True)
     lib = ctypes.CDLL(FASTMATH)
-    t = decode.bc6h_srgb8_thresholds()
+    b = decode.bc6h_srgb8_buckets()
     n = 0x7C00
-    est = np.zeros(n, np.uint8)
     code = np.zeros(n, np.uint8)
-    rc = lib.srgb8_check(t.ctypes.data_as(ctypes.c_void_p), ctypes.c_uint32(n),
-                         est.ctypes.data_as(ctypes.c_void_p), code.ctypes.data_as(ctypes.c_void_p))
+    rc = lib.srgb8_check(b.ctypes.data_as(ctypes.c_void_p), ctypes.c_uint32(n),
+                         code.ctypes.data_as(ctypes.c_void_p))
     assert rc == 0
     want = oracle_lib.linear_to_srgb(np.arange(n, dtype=np.uint16).view(np.float16).astype(np.float32))
-    assert np.abs(est.astype(int) - want.astype(int)).max() <= 1
     assert np.array_equal(code, want)
 
 
@@ -79,13 +77,17 @@ def test_bc6h_rgba8_matches_oracle(source):
     assert np.array_equal(out.cpu().numpy(), want)
 
 
+# (96, 64): 24 blocks per row, every wave spans rows (per-lane stores);
+# (512, 16): whole waves inside one block row (row-run stores through LDS);
+# (320, 8): 80 blocks per row, one wave of each kind, and a partial last wave.
+@pytest.mark.parametrize("size", [(96, 64), (512, 16), (320, 8)])
 @pytest.mark.parametrize("kind", ["16f", "32f"])
 @pytest.mark.parametrize("source", ["random", "encoded"])
-def test_bc6h_float_images_match_oracle(kind, source):
+def test_bc6h_float_images_match_oracle(kind, source, size):
     from block_compression_amd import decode
 
-    print(f"decode-image: BC6H rgba{kind} {source}", flush=True)
-    w, h = 96, 64
+    print(f"decode-image: BC6H rgba{kind} {source} {size}", flush=True)
+    w, h = size
     if source == "random":
         blocks = _dev(np.random.default_rng(37).integers(0, 256, V6.blocks_byte_size(w, h), dtype=np.uint8))
     else:

@@ -115,3 +115,56 @@ def test_bc7_decoder_covers_every_mode():
     blocks[:5, 0] = 0  # invalid mode: all zero
     got = kc.decode(6, blocks.reshape(-1))
     assert np.array_equal(got, _oracle_decode_rgba8("bc7", blocks.reshape(-1), 16))
+
+
+def test_block_pos_equals_division():
+    """The decoders' division-free row/column (block_pos) against divmod on
+    every width class it is used for: widths 1..2^24-1 (block counts up to
+    2^31), indices at both ends of rows and at the largest allowed index."""
+    rng = np.random.default_rng(5)
+    bws = np.concatenate([np.arange(1, 2049), [4095, 4096, 4097, 65535, 65536, 65537, (1 << 24) - 1],
+                          rng.integers(1, 1 << 24, 2000)]).astype(np.int64)
+    bw, idx = [], []
+    for w in bws:
+        rows = min((1 << 31) // w, (1 << 24) - 1)  # the entry points' limits
+        top = rows * w - 1
+        cand = [0, w - 1, w, min(w + 1, top), top, max(top - w, 0), (rows // 2) * w, (rows // 2) * w - 1]
+        cand += list(rng.integers(0, top + 1, 4))
+        for i in cand:
+            if 0 <= i <= top:
+                bw.append(w)
+                idx.append(i)
+    bw, idx = np.array(bw, np.int64), np.array(idx, np.int64)
+    rows, cols = kc.block_pos(bw, idx)
+    assert np.array_equal(rows.astype(np.int64), idx // bw)
+    assert np.array_equal(cols.astype(np.int64), idx % bw)
+
+
+def test_channel_decoder_every_endpoint_pair():
+    """BC4 blocks with every (a0, a1) and texel k on index k % 8: all 8
+    levels of all 65,536 palettes equal block.rs:219-258's integer forms,
+    ((7 - i) a0 + i a1) / 7 for a0 > a1 and ((5 - i) a0 + i a1) / 5 with 0
+    and 255 otherwise, and the oracle's decoder on a sample."""
+    a0, a1 = np.meshgrid(np.arange(256), np.arange(256), indexing="ij")
+    a0, a1 = a0.reshape(-1), a1.reshape(-1)
+    bits = 0
+    for k in range(16):
+        bits |= (k % 8) << (3 * k)
+    idx = np.array([(bits >> (8 * b)) & 0xFF for b in range(6)], np.uint8)
+    blocks = np.zeros((65536, 8), np.uint8)
+    blocks[:, 0], blocks[:, 1], blocks[:, 2:] = a0, a1, idx
+    words = kc.decode(3, blocks.reshape(-1)).reshape(-1, 16).astype(np.int64)  # (R, 0, 0, 0)
+    assert not (words >> 8).any()
+    got = words & 0xFF
+    lv = np.zeros((65536, 8), np.int64)
+    lv[:, 0], lv[:, 1] = a0, a1
+    six = a0 > a1
+    for i in range(1, 7):
+        lv[six, 1 + i] = ((7 - i) * a0[six] + i * a1[six]) // 7
+    for i in range(1, 5):
+        lv[~six, 1 + i] = ((5 - i) * a0[~six] + i * a1[~six]) // 5
+    lv[~six, 6], lv[~six, 7] = 0, 255
+    assert np.array_equal(got, lv[:, np.arange(16) % 8])
+    sample = np.random.default_rng(3).choice(65536, 512, replace=False)
+    ref = np.stack([oracle_lib.decode_block("bc4", blocks[i].tobytes(), 4, 16) for i in sample])
+    assert np.array_equal(got[sample], ref.astype(np.int64))
