@@ -65,6 +65,33 @@ __device__ __forceinline__ void load_bc6h_fields(Bc6hLds& l) {
   for (uint32_t i = threadIdx.x; i < kBc6hDecRows * 23; i += kDecGroup) l.fields[i] = (&kBc6hDec.f[0][0])[i];
 }
 
+// One lane's BC6H block to RGBA8 sRGB codes, row by row: each row is stored
+// as soon as it is decoded.  ONE: the wave holds only one-subset blocks.
+template <bool SGN, bool ONE>
+__device__ __forceinline__ void bc6h_srgb8_rows(const uint32_t in[4], int m, Bc6hScratch sc,
+                                                const uint16_t* fields, const uint16_t* lb,
+                                                uint8_t* o, uint32_t pitch) {
+  Bc6hIdx st = decode_bc6h_header<SGN, ONE>(in, m, sc, fields);
+  // Negative values (and -0) code as 0, the code of +0.
+  auto word = [&](uint16_t v) -> uint32_t {
+    const uint32_t x = SGN ? (uint32_t)max((int)(int16_t)v, 0) : (uint32_t)v;
+    return srgb8_word(x, lb);
+  };
+#pragma unroll
+  for (int y = 0; y < 4; y++) {
+    uint32_t px[4];
+#pragma unroll
+    for (int x = 0; x < 4; x++) {
+      uint16_t h[3];
+      decode_bc6h_texel<SGN, ONE>(st, y * 4 + x, sc, h);
+      // The high bytes of the three words: R, G, then B; alpha 0.
+      const uint32_t rg = perm_b32(word(h[1]), word(h[0]), 0x0C0C0501u);
+      px[x] = perm_b32(word(h[2]), rg, 0x0C050100u);
+    }
+    *reinterpret_cast<uint4*>(o + (size_t)y * pitch) = make_uint4(px[0], px[1], px[2], px[3]);
+  }
+}
+
 // BC6H -> RGBA8 (decode.rs:161-181): unsigned decode, linear_to_srgb per
 // channel, alpha 0.  SGN: signed blocks, whose negative values (and -0) map
 // to code 0 as linear_to_srgb clamps them.  Each workgroup copies the sRGB8
@@ -84,29 +111,14 @@ __global__ __launch_bounds__(kDecGroup) void decode_bc6h_srgb8_kernel(
   const uint32_t* src = reinterpret_cast<const uint32_t*>(blocks + (size_t)idx * 16);
   const uint32_t in[4] = {src[0], src[1], src[2], src[3]};
   const Bc6hScratch sc{lds.scratch + threadIdx.x, kDecGroup};
-  Bc6hIdx st = decode_bc6h_header<SGN>(in, sc, lds.fields);
   uint32_t yy, xx;
   block_pos(rows, idx, yy, xx);
   uint8_t* o = rgba + (size_t)yy * 4 * pitch + (size_t)xx * 16;
-  // Negative values (and -0) code as 0, the code of +0.
-  auto word = [&](uint16_t v) -> uint32_t {
-    const uint32_t x = SGN ? (uint32_t)max((int)(int16_t)v, 0) : (uint32_t)v;
-    return srgb8_word(x, lb);
-  };
-  // Row by row: each row is stored as soon as it is decoded.
-#pragma unroll
-  for (int y = 0; y < 4; y++) {
-    uint32_t px[4];
-#pragma unroll
-    for (int x = 0; x < 4; x++) {
-      uint16_t h[3];
-      decode_bc6h_texel<SGN>(st, y * 4 + x, sc, h);
-      // The high bytes of the three words: R, G, then B; alpha 0.
-      const uint32_t rg = perm_b32(word(h[1]), word(h[0]), 0x0C0C0501u);
-      px[x] = perm_b32(word(h[2]), rg, 0x0C050100u);
-    }
-    *reinterpret_cast<uint4*>(o + (size_t)y * pitch) = make_uint4(px[0], px[1], px[2], px[3]);
-  }
+  const int m = bc6h_mode(in);
+  if (wave_all(bc6h_one_subset(m)))
+    bc6h_srgb8_rows<SGN, true>(in, m, sc, lds.fields, lb, o, pitch);
+  else
+    bc6h_srgb8_rows<SGN, false>(in, m, sc, lds.fields, lb, o, pitch);
 }
 
 // One block row per lane (N x 16 bytes, lane order) stored through the
@@ -132,44 +144,23 @@ __device__ __forceinline__ void store_row_run(const uint4 (&v)[N], uint4* slice,
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int OUT, bool SGN>
-__global__ __launch_bounds__(kDecGroup) void decode_bc6h_kernel(const uint8_t* __restrict__ blocks,
-                                                                BlockRows rows, uint32_t nblocks,
-                                                                uint8_t* __restrict__ rgba,
-                                                                uint32_t pitch) {
-  __shared__ Bc6hLds lds;
-  // RGBA outputs: each wave's row-run slice, 64 rows of 32 (16F) or 64 (32F)
-  // bytes.
-  constexpr bool F32 = OUT == kOutF32Rgba || OUT == kOutF32RefRgb;
+// One lane's BC6H block to the float outputs, row by row: each row is
+// stored as soon as it is decoded, through the wave's row-run slice when
+// `run` (the wave's 64 block rows consecutive from run_dst).  ONE: the wave
+// holds only one-subset blocks.
+template <int OUT, bool SGN, bool ONE, int N>
+__device__ __forceinline__ void bc6h_float_rows(const uint32_t in[4], int m, Bc6hScratch sc,
+                                                const uint16_t* fields, uint8_t* o, uint32_t pitch,
+                                                bool run, uint4* slice, uint8_t* run_dst,
+                                                uint32_t lane) {
   constexpr bool kRun = OUT == kOutF16Rgba || OUT == kOutF32Rgba;
-  constexpr int N = F32 ? 4 : 2;  // 16-byte chunks per block row
-  __shared__ uint4 runs[kRun ? (kDecGroup / 64) * 64 * N : 1];
-  load_bc6h_fields(lds);
-  __syncthreads();
-  const uint32_t idx = blockIdx.x * kDecGroup + threadIdx.x;
-  if (idx >= nblocks) return;
-  const uint32_t* src = reinterpret_cast<const uint32_t*>(blocks + (size_t)idx * 16);
-  const uint32_t in[4] = {src[0], src[1], src[2], src[3]};
-  const Bc6hScratch sc{lds.scratch + threadIdx.x, kDecGroup};
-  Bc6hIdx st = decode_bc6h_header<SGN>(in, sc, lds.fields);
-  uint32_t yy, xx;
-  block_pos(rows, idx, yy, xx);
-  uint8_t* o = rgba + (size_t)yy * 4 * pitch + (size_t)xx * (F32 ? 64 : 32);
-  // Whether the wave's 64 blocks all exist and sit in one block row: then its
-  // rows are consecutive in memory and go out through the row-run slice.
-  const uint32_t lane = threadIdx.x & 63u, first = idx - lane;
-  uint32_t fy, fx;
-  block_pos(rows, first, fy, fx);
-  const bool run = kRun && first + 63 < nblocks && fx + 63 < rows.bw;
-  uint4* slice = runs + (threadIdx.x >> 6) * 64 * N;
-  uint8_t* run_dst = rgba + (size_t)fy * 4 * pitch + (size_t)fx * (F32 ? 64 : 32);
-  // Row by row: each row is stored as soon as it is decoded.
+  Bc6hIdx st = decode_bc6h_header<SGN, ONE>(in, m, sc, fields);
 #pragma unroll
   for (int y = 0; y < 4; y++) {
     uint8_t* r = o + (size_t)y * pitch;
     uint16_t hr[12];  // the row's 4 texels, RGB interleaved
 #pragma unroll
-    for (int x = 0; x < 4; x++) decode_bc6h_texel<SGN>(st, y * 4 + x, sc, hr + 3 * x);
+    for (int x = 0; x < 4; x++) decode_bc6h_texel<SGN, ONE>(st, y * 4 + x, sc, hr + 3 * x);
     if (kRun) {
       uint4 v[N];
 #pragma unroll
@@ -213,6 +204,43 @@ __global__ __launch_bounds__(kDecGroup) void decode_bc6h_kernel(const uint8_t* _
                             __float_as_uint(half_to_float(hr[4 * j + 3])));
     }
   }
+}
+
+template <int OUT, bool SGN>
+__global__ __launch_bounds__(kDecGroup) void decode_bc6h_kernel(const uint8_t* __restrict__ blocks,
+                                                                BlockRows rows, uint32_t nblocks,
+                                                                uint8_t* __restrict__ rgba,
+                                                                uint32_t pitch) {
+  __shared__ Bc6hLds lds;
+  // RGBA outputs: each wave's row-run slice, 64 rows of 32 (16F) or 64 (32F)
+  // bytes.
+  constexpr bool F32 = OUT == kOutF32Rgba || OUT == kOutF32RefRgb;
+  constexpr bool kRun = OUT == kOutF16Rgba || OUT == kOutF32Rgba;
+  constexpr int N = F32 ? 4 : 2;  // 16-byte chunks per block row
+  __shared__ uint4 runs[kRun ? (kDecGroup / 64) * 64 * N : 1];
+  load_bc6h_fields(lds);
+  __syncthreads();
+  const uint32_t idx = blockIdx.x * kDecGroup + threadIdx.x;
+  if (idx >= nblocks) return;
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(blocks + (size_t)idx * 16);
+  const uint32_t in[4] = {src[0], src[1], src[2], src[3]};
+  const Bc6hScratch sc{lds.scratch + threadIdx.x, kDecGroup};
+  uint32_t yy, xx;
+  block_pos(rows, idx, yy, xx);
+  uint8_t* o = rgba + (size_t)yy * 4 * pitch + (size_t)xx * (F32 ? 64 : 32);
+  // Whether the wave's 64 blocks all exist and sit in one block row: then its
+  // rows are consecutive in memory and go out through the row-run slice.
+  const uint32_t lane = threadIdx.x & 63u, first = idx - lane;
+  uint32_t fy, fx;
+  block_pos(rows, first, fy, fx);
+  const bool run = kRun && first + 63 < nblocks && fx + 63 < rows.bw;
+  uint4* slice = runs + (threadIdx.x >> 6) * 64 * N;
+  uint8_t* run_dst = rgba + (size_t)fy * 4 * pitch + (size_t)fx * (F32 ? 64 : 32);
+  const int m = bc6h_mode(in);
+  if (wave_all(bc6h_one_subset(m)))
+    bc6h_float_rows<OUT, SGN, true, N>(in, m, sc, lds.fields, o, pitch, run, slice, run_dst, lane);
+  else
+    bc6h_float_rows<OUT, SGN, false, N>(in, m, sc, lds.fields, o, pitch, run, slice, run_dst, lane);
 }
 
 // ---- host side --------------------------------------------------------------

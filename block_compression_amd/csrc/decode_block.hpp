@@ -454,49 +454,74 @@ BCN_HD bool wave_any(bool p) {
 #endif
 }
 
+// Whether every lane of the wave has `p` (the lane's own value on the host).
+BCN_HD bool wave_all(bool p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __ballot(!p) == 0;
+#else
+  return p;
+#endif
+}
+
+// Mode index of a BC6H block (0..13; 14 for the reserved prefixes) from its
+// 2- or 5-bit prefix (block.rs:270-290).
+BCN_HD int bc6h_mode(const uint32_t in[4]) {
+  const uint32_t b = in[0] & 31u;
+  const uint32_t mb = (b & 3u) > 1 ? b : (b & 3u);
+  constexpr uint32_t kM0 = bc6h_mode_nibbles(0), kM1 = bc6h_mode_nibbles(1), kM2 = bc6h_mode_nibbles(2),
+                     kM3 = bc6h_mode_nibbles(3);
+  const uint32_t mw = mb >> 3;
+  const uint32_t mk = mw == 0 ? kM0 : (mw == 1 ? kM1 : (mw == 2 ? kM2 : kM3));
+  const int m1 = (int)bfe_u32(mk, 4u * (mb & 7u), 4);
+  return m1 ? m1 - 1 : kBc6hDecRows - 1;
+}
+
+// Whether mode index m has one subset (modes 10..13, and the reserved row).
+BCN_HD bool bc6h_one_subset(int m) { return m >= 10; }
+
 // BC6H index state after the block's header: the remaining index bits and
-// what the texels need to read them (decode_bc6h_texel reads them in order).
+// what the texels need to read them (decode_bc6h_texel reads them in order),
+// and, for one-subset blocks (ONE), the subset's interpolation records.
 struct Bc6hIdx {
   uint32_t j0, j1;   // index bits not yet read, lowest first
   uint32_t amask;    // anchor texels (read one bit less)
   uint32_t pattern;  // 2-bit subset ids of the 16 texels
   uint32_t ib;       // index bits
   int kw;            // weight multiplier: w = (i * kw + 32) >> 6
+  uint32_t w2, w3;   // ONE: the block's last two words (the indices)
+  int c[3], d[3];    // ONE: the records of the one subset
 };
 
-// decode_block_bc6h (block.rs:262-769), the header: mode, endpoint fields,
-// transforms and unquantize, leaving each subset's interpolation records in
-// the lane's scratch row (words 6..17) and returning the index state.  Every
-// mode runs one branch-free path: field widths, destinations and precisions
-// come from tables indexed by the lane's mode, and a field the mode lacks is
-// a read of width 0.
+// decode_block_bc6h (block.rs:262-769), the header of a block of mode index
+// m (bc6h_mode): endpoint fields, transforms and unquantize, leaving each
+// subset's interpolation records in the lane's scratch row (words 6..17)
+// and returning the index state.  Every mode runs one branch-free path:
+// field widths, destinations and precisions come from tables indexed by the
+// lane's mode, and a field the mode lacks is a read of width 0.
 //   * Each endpoint field is read and ORed into its 16-bit slot in the lane's
 //     scratch row (an LDS OR on the GPU), rather than merged through selects
 //     over all 12 slots.  `fields` is kBc6hDec (a copy in LDS on the GPU).
+//     The field loop ends once no lane of the wave has a field left.
 //   * A reserved mode prefix takes the field row of no fields: its
 //     components are all 0 and so is every texel (block.rs:282-290).
+//   * ONE: every lane of the wave holds a one-subset block (at most 9
+//     fields, 2 endpoints, 4-bit indices); the records stay in registers.
 //   * The indices end the block, so they are read from its top 64 bits;
 //     anchor texels read one bit less, from an anchor mask.
 //   * Interpolation lo + ((w (hi - lo) + 32) >> 6) is (c + w d) >> 6 with
 //     c = 64 lo + 32 (64 lo is a multiple of 64): one 24-bit multiply-add.
-template <bool SIGNED>
-BCN_HD Bc6hIdx decode_bc6h_header(const uint32_t in[4], Bc6hScratch sc, const uint16_t* fields) {
+template <bool SIGNED, bool ONE>
+BCN_HD Bc6hIdx decode_bc6h_header(const uint32_t in[4], int m, Bc6hScratch sc, const uint16_t* fields) {
   BitReader br(in);
-  uint32_t mb = br.read(2);
-  mb |= br.read(mb > 1 ? 3u : 0u) << 2;
-  constexpr uint32_t kM0 = bc6h_mode_nibbles(0), kM1 = bc6h_mode_nibbles(1), kM2 = bc6h_mode_nibbles(2),
-                     kM3 = bc6h_mode_nibbles(3);
-  const uint32_t mw = mb >> 3;
-  const uint32_t mk = mw == 0 ? kM0 : (mw == 1 ? kM1 : (mw == 2 ? kM2 : kM3));
-  const int m1 = (int)bfe_u32(mk, 4u * (mb & 7u), 4);
-  const int m = m1 ? m1 - 1 : kBc6hDecRows - 1;
+  br.skip((in[0] & 3u) > 1 ? 5u : 2u);
 #pragma unroll
   for (int w = 0; w < 6; w++) sc.at(w) = 0;
   const uint16_t* ft = fields + m * 23;
   // Every field is merged as read; the reversed ones are put in order below.
 #pragma unroll
-  for (int i = 0; i < 23; i++) {
+  for (int i = 0; i < (ONE ? 9 : 23); i++) {
     const uint32_t t = ft[i];
+    if (!ONE && !wave_any(t != 0u)) break;  // entries past a mode's count are 0
     const uint32_t n = t & 31u, dst = (t >> 5) & 0xFFu;
     const uint32_t v = br.read(n);
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -506,9 +531,10 @@ BCN_HD Bc6hIdx decode_bc6h_header(const uint32_t in[4], Bc6hScratch sc, const ui
     sc.at((int)(dst >> 5)) |= v << (dst & 31u);
 #endif
   }
+  constexpr int kComp = ONE ? 6 : 12;  // components in use
   int f[12];
 #pragma unroll
-  for (int w = 0; w < 6; w++) {
+  for (int w = 0; w < kComp / 2; w++) {
     const uint32_t x = sc.at(w);
     f[2 * w] = (int)(x & 0xFFFFu);
     f[2 * w + 1] = (int)(x >> 16);
@@ -522,8 +548,8 @@ BCN_HD Bc6hIdx decode_bc6h_header(const uint32_t in[4], Bc6hScratch sc, const ui
     const uint32_t x = (uint32_t)f[c];
     f[c] = nrev ? (int)((x & 0x3FFu) | (reverse_low_bits(x >> 10, nrev) << 10)) : f[c];
   }
-  const bool two = m < 10;
-  const uint32_t part = br.read(two ? 5u : 0u);
+  const bool two = !ONE && m < 10;
+  const uint32_t part = ONE ? 0u : br.read(two ? 5u : 0u);
   const int ne = two ? 4 : 2;
   // Base bits (6..16) and the delta bits of R, G, B (kBc6hPrec), minus one,
   // as nibbles: modes 0..7 in the low word (the reserved row reads 0).
@@ -538,7 +564,7 @@ BCN_HD Bc6hIdx decode_bc6h_header(const uint32_t in[4], Bc6hScratch sc, const ui
 #pragma unroll
     for (int c = 0; c < 3; c++) f[c] = extend_sign(f[c], b0);
 #pragma unroll
-  for (int e = 1; e < 4; e++)
+  for (int e = 1; e < kComp / 3; e++)
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       int v = f[e * 3 + c];
@@ -552,18 +578,25 @@ BCN_HD Bc6hIdx decode_bc6h_header(const uint32_t in[4], Bc6hScratch sc, const ui
       f[e * 3 + c] = e < ne ? v : f[e * 3 + c];
     }
 #pragma unroll
-  for (int i = 0; i < 12; i++) f[i] = bc6h_unquantize<SIGNED>(f[i], b0);
+  for (int i = 0; i < kComp; i++) f[i] = bc6h_unquantize<SIGNED>(f[i], b0);
   // Subset records: c = 64 lo + 32 and d = hi - lo per channel (|d| < 2^17,
   // 64 |d| + c < 2^23: one v_mad_i32_i24).
+  Bc6hIdx st;
 #pragma unroll
-  for (int sb = 0; sb < 2; sb++)
+  for (int sb = 0; sb < kComp / 6; sb++)
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       const int lo = f[6 * sb + c], hi = f[6 * sb + 3 + c];
-      sc.at(6 + 6 * sb + c) = (uint32_t)(64 * lo + 32);
-      sc.at(6 + 6 * sb + 3 + c) = (uint32_t)(hi - lo);
+      if (ONE) {
+        st.c[c] = 64 * lo + 32;
+        st.d[c] = hi - lo;
+      } else {
+        sc.at(6 + 6 * sb + c) = (uint32_t)(64 * lo + 32);
+        sc.at(6 + 6 * sb + 3 + c) = (uint32_t)(hi - lo);
+      }
     }
-  Bc6hIdx st;
+  st.w2 = in[2];
+  st.w3 = in[3];
   st.pattern = two ? kPart.subset[part] : 0u;
   const uint32_t a1 = two ? kPart.anchor1[part] : 0u;
   st.amask = 1u | (1u << a1);
@@ -576,40 +609,54 @@ BCN_HD Bc6hIdx decode_bc6h_header(const uint32_t in[4], Bc6hScratch sc, const ui
   return st;
 }
 
-// Texel k's RGB half bit patterns; texels are read in order 0..15.
-template <bool SIGNED>
+// Texel k's RGB half bit patterns; texels are read in order 0..15.  ONE:
+// the 4-bit indices of texels 1..15 are the nibbles 1..15 of the block's
+// last 64 bits (texel 0, the anchor, is bits 65..67).
+template <bool SIGNED, bool ONE>
 BCN_HD void decode_bc6h_texel(Bc6hIdx& st, int k, Bc6hScratch sc, uint16_t rgb[3]) {
-  const uint32_t n = st.ib - bfe_u32(st.amask, (uint32_t)k, 1);
-  const uint32_t x = bfe_u32(st.j0, 0, n);
-  st.j0 = funnel_shr(st.j1, st.j0, n);
-  st.j1 >>= n;
-  const int w = (mul24((int)x, st.kw) + 32) >> 6;
-  const int s = (int)bfe_u32(st.pattern, 2u * (uint32_t)k, 1);
+  uint32_t x;
+  int s = 0;
+  if (ONE) {
+    x = k == 0 ? bfe_u32(st.w2, 1, 3) : bfe_u32(k < 8 ? st.w2 : st.w3, 4u * (uint32_t)(k & 7), 4);
+  } else {
+    const uint32_t n = st.ib - bfe_u32(st.amask, (uint32_t)k, 1);
+    x = bfe_u32(st.j0, 0, n);
+    st.j0 = funnel_shr(st.j1, st.j0, n);
+    st.j1 >>= n;
+    s = (int)bfe_u32(st.pattern, 2u * (uint32_t)k, 1);
+  }
+  const int w = (mul24((int)x, ONE ? 273 : st.kw) + 32) >> 6;
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    const int cc = (int)sc.at(6 + 6 * s + c), dd = (int)sc.at(6 + 6 * s + 3 + c);
+    const int cc = ONE ? st.c[c] : (int)sc.at(6 + 6 * s + c);
+    const int dd = ONE ? st.d[c] : (int)sc.at(6 + 6 * s + 3 + c);
     rgb[c] = bc6h_finish<SIGNED>((mul24i(w, dd) + cc) >> 6);
   }
 }
 
-// The whole block: out[k*3 + c] for texel k, channel c.
-template <bool SIGNED>
+// The whole block: out[k*3 + c] for texel k, channel c.  ONE may be set only
+// for one-subset blocks (bc6h_one_subset); the general path takes any block.
+template <bool SIGNED, bool ONE = false>
 BCN_HD void decode_bc6h_t(const uint32_t in[4], uint16_t out[48], Bc6hScratch sc,
                           const uint16_t* fields) {
-  Bc6hIdx st = decode_bc6h_header<SIGNED>(in, sc, fields);
+  const int m = bc6h_mode(in);
+  Bc6hIdx st = decode_bc6h_header<SIGNED, ONE>(in, m, sc, fields);
 #pragma unroll
-  for (int k = 0; k < 16; k++) decode_bc6h_texel<SIGNED>(st, k, sc, out + 3 * k);
+  for (int k = 0; k < 16; k++) decode_bc6h_texel<SIGNED, ONE>(st, k, sc, out + 3 * k);
 }
 
 // Host form with private scratch (tests and the CPU kernel-check build).
-BCN_HD void decode_bc6h(const uint32_t in[4], bool is_signed, uint16_t out[48]) {
+// one_path: one-subset blocks take the ONE path (as in a wave of only such
+// blocks); otherwise every block takes the general path.
+BCN_HD void decode_bc6h(const uint32_t in[4], bool is_signed, uint16_t out[48], bool one_path = false) {
   uint32_t row[18];
   const Bc6hScratch sc{row, 1};
   const uint16_t* ft = &kBc6hDec.f[0][0];
+  const bool one = one_path && bc6h_one_subset(bc6h_mode(in));
   if (is_signed)
-    decode_bc6h_t<true>(in, out, sc, ft);
+    one ? decode_bc6h_t<true, true>(in, out, sc, ft) : decode_bc6h_t<true, false>(in, out, sc, ft);
   else
-    decode_bc6h_t<false>(in, out, sc, ft);
+    one ? decode_bc6h_t<false, true>(in, out, sc, ft) : decode_bc6h_t<false, false>(in, out, sc, ft);
 }
 
 // ---- sRGB8 codes of BC6H texels (linear_to_srgb, src/decode.rs:151-159) -----

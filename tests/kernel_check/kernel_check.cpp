@@ -137,7 +137,8 @@ extern "C" int kcheck_bc6h_encode(const void* settings, const uint16_t* rgba16, 
 
 // Decoders: `count` blocks of `bb` bytes -> 16 texels each.  fmt 0..4, 6
 // write RGBA8 texels (BC4 -> R,0,0,0; BC5 -> R,G,0,0); fmt 5 writes 48 f16
-// bit patterns (RGB per texel).
+// bit patterns (RGB per texel), signed if bit 0 of is_signed is set, with
+// one-subset blocks through the one-subset path if bit 1 is.
 extern "C" int kcheck_decode(int fmt, int is_signed, const uint8_t* blocks, size_t count, void* out) {
   size_t bb = (fmt == 0 || fmt == 3) ? 8 : 16;
   for (size_t i = 0; i < count; i++) {
@@ -145,7 +146,7 @@ extern "C" int kcheck_decode(int fmt, int is_signed, const uint8_t* blocks, size
     std::memcpy(in, blocks + i * bb, bb);
     if (fmt == 5) {
       uint16_t h[48];
-      bcn::decode_bc6h(in, is_signed != 0, h);
+      bcn::decode_bc6h(in, (is_signed & 1) != 0, h, (is_signed & 2) != 0);
       std::memcpy((uint16_t*)out + i * 48, h, sizeof(h));
       continue;
     }

@@ -69,12 +69,14 @@ def bc6h_encode(settings: bytes, img16: np.ndarray, is_signed: bool = False) -> 
     return out
 
 
-def decode(fmt: int, blocks: np.ndarray, is_signed: bool = False) -> np.ndarray:
+def decode(fmt: int, blocks: np.ndarray, is_signed: bool = False, one_path: bool = False) -> np.ndarray:
+    """one_path (BC6H): one-subset blocks through the decoder's one-subset
+    path, as in a wave of only such blocks."""
     blocks = np.ascontiguousarray(blocks, dtype=np.uint8)
     bb = 8 if fmt in (0, 3) else 16
     n = blocks.size // bb
     out = np.zeros(n * 48, np.uint16) if fmt == 5 else np.zeros(n * 16, np.uint32)
-    lib().kcheck_decode(fmt, int(is_signed), _p(blocks), n, _p(out))
+    lib().kcheck_decode(fmt, int(is_signed) | (2 if one_path else 0), _p(blocks), n, _p(out))
     return out
 
 

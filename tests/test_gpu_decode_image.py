@@ -150,3 +150,43 @@ def test_bc6h_2048_float_and_srgb_consistent():
     for by in rows:
         want = oracle_lib.decompress_rgba8(5, w, 4, nb[by])
         assert np.array_equal(got[by * 4:by * 4 + 4], want)
+
+
+@pytest.mark.parametrize("is_signed", [False, True])
+def test_bc6h_one_subset_waves_match_oracle(is_signed):
+    """Waves of only one-subset blocks take the decoder's one-subset path,
+    mixed waves the general one: random blocks with one-subset prefixes
+    (modes 11-14 and a reserved prefix) in the first 4 waves of each row,
+    mixed blocks after, decoded to RGBA16F, RGBA32F and RGBA8 and compared
+    with the oracle texel for texel."""
+    from block_compression_amd import decode
+
+    print(f"decode-image: BC6H one-subset waves signed={is_signed}", flush=True)
+    v = CompressionVariant.BC6H_SF16(BC6HSettings.basic()) if is_signed else V6
+    w, h = 1024, 8  # 256 blocks per row: 4 waves
+    rng = np.random.default_rng(43 + is_signed)
+    blocks = rng.integers(0, 256, (h // 4, w // 4, 16), dtype=np.uint8)
+    one = np.array([3, 7, 11, 15, 19], np.uint8)  # 19: a reserved prefix
+    blocks[:, :192, 0] = (blocks[:, :192, 0] & ~np.uint8(31)) | one[rng.integers(0, 5, (h // 4, 192))]
+    nb = blocks.reshape(-1)
+    want16 = np.stack([oracle_lib.decode_block_bc6h(b.tobytes(), 12, is_signed)[:48]
+                       for b in blocks.reshape(-1, 16)]).reshape(h // 4, w // 4, 4, 4, 3)
+    want16 = want16.transpose(0, 2, 1, 3, 4).reshape(h, w, 3)
+    dev = _dev(nb)
+    f16 = torch.empty((h, w, 4), dtype=torch.float16, device="cuda")
+    f32 = torch.empty((h, w, 4), dtype=torch.float32, device="cuda")
+    s8 = torch.empty((h, w, 4), dtype=torch.uint8, device="cuda")
+    decode.decompress_blocks_as_rgba16f(v, w, h, dev, f16)
+    decode.decompress_blocks_as_rgba32f(v, w, h, dev, f32)
+    decode.decompress_blocks_as_rgba8(v, w, h, dev, s8)
+    torch.cuda.synchronize()
+    g16 = f16.cpu().numpy().view(np.uint16)
+    assert np.array_equal(g16[..., :3], want16)
+    assert torch.equal(f16.float(), f32)
+    # linear_to_srgb of each channel (negative values clamp to 0), alpha 0.
+    lin = want16.view(np.float16).astype(np.float32)
+    want8 = np.zeros((h, w, 4), np.uint8)
+    want8[..., :3] = oracle_lib.linear_to_srgb(lin.reshape(-1)).reshape(h, w, 3)
+    assert np.array_equal(s8.cpu().numpy(), want8)
+    if not is_signed:
+        assert np.array_equal(want8, oracle_lib.decompress_rgba8(5, w, h, nb))

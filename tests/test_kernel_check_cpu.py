@@ -98,12 +98,18 @@ def test_bc4_bc5_decoders_match_oracle_on_random_blocks():
     assert np.array_equal((got >> 8) & 0xFF, ref[:, 1::2].astype(np.uint32))
 
 
+@pytest.mark.parametrize("one_path", [False, True])
 @pytest.mark.parametrize("is_signed", [False, True])
-def test_bc6h_decoder_matches_oracle_on_random_blocks(is_signed):
-    blocks = np.random.default_rng(6 + is_signed).integers(0, 256, 4000 * 16, dtype=np.uint8)
-    got = kc.decode(5, blocks, is_signed).reshape(-1, 48)
-    ref = np.stack([oracle_lib.decode_block_bc6h(b.tobytes(), 12, is_signed)[:48]
-                    for b in blocks.reshape(-1, 16)])
+def test_bc6h_decoder_matches_oracle_on_random_blocks(is_signed, one_path):
+    """Random blocks (every mode, reserved prefixes included) through the
+    general path, and with the one-subset blocks through the one-subset
+    path; one in four blocks is forced to a one-subset prefix so that path
+    sees ~2,000 blocks."""
+    blocks = np.random.default_rng(6 + is_signed).integers(0, 256, (4000, 16), dtype=np.uint8)
+    blocks[::4, 0] = (blocks[::4, 0] & ~np.uint8(31)) | np.array([3, 7, 11, 15], np.uint8)[
+        np.arange(1000) % 4]
+    got = kc.decode(5, blocks.reshape(-1), is_signed, one_path).reshape(-1, 48)
+    ref = np.stack([oracle_lib.decode_block_bc6h(b.tobytes(), 12, is_signed)[:48] for b in blocks])
     assert np.array_equal(got, ref)
 
 
