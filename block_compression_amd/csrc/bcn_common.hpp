@@ -147,13 +147,16 @@ BCN_HD uint32_t perm_b32(uint32_t src0, uint32_t src1, uint32_t sel) {
 #endif
 }
 
-// Bits off .. off + w - 1 of x (v_bfe_u32), for off < 32 and off + w <= 32;
-// w = 0 gives 0.
+// Bits off .. off + w - 1 of x (v_bfe_u32): the instruction takes off and w
+// modulo 32, w = 0 gives 0, and bits past bit 31 read as 0.  The host form
+// computes the same for every input.
 BCN_HD uint32_t bfe_u32(uint32_t x, uint32_t off, uint32_t w) {
 #if defined(__HIP_DEVICE_COMPILE__)
   return __builtin_amdgcn_ubfe(x, off, w);
 #else
-  return w == 0 ? 0u : (x >> off) & (w >= 32 ? 0xFFFFFFFFu : ((1u << w) - 1u));
+  off &= 31u;
+  w &= 31u;
+  return w == 0 ? 0u : (x >> off) & ((1u << w) - 1u);
 #endif
 }
 

@@ -9,14 +9,17 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "kernel_check", "_build", "libkernel_check.so")
+# KC_LIB names another build of kernel_check.cpp (the sanitizer build of
+# test_sanitizers_cpu.py); default: the plain build, made on first use.
+LIB = os.environ.get("KC_LIB") or os.path.join(HERE, "kernel_check", "_build", "libkernel_check.so")
 _lib = None
 
 
 def lib():
     global _lib
     if _lib is None:
-        subprocess.run([os.path.join(HERE, "kernel_check", "build.sh")], check=True)
+        if not os.environ.get("KC_LIB"):
+            subprocess.run([os.path.join(HERE, "kernel_check", "build.sh")], check=True)
         _lib = ctypes.CDLL(LIB)
     return _lib
 
