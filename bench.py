@@ -87,8 +87,8 @@ def cpu_baseline_leg(img, settings, nblocks: int, threads: int, host: dict) -> d
 
     rng = np.random.default_rng(7)
     many = np.sort(rng.choice(nblocks, 65536, replace=False))
-    one = np.sort(rng.choice(nblocks, 4096, replace=False))
-    log(f"bench: CPU baseline, {threads} threads (65,536 blocks) and 1 thread (4,096 blocks), median of 5")
+    one = np.sort(rng.choice(nblocks, 16384, replace=False))
+    log(f"bench: CPU baseline, {threads} threads (65,536 blocks) and 1 thread (16,384 blocks), median of 5")
     v_all, t_all, runs_all = median_rate(many, threads)
     v_one, t_one, runs_one = median_rate(one, 1)
     return {
@@ -101,7 +101,7 @@ def cpu_baseline_leg(img, settings, nblocks: int, threads: int, host: dict) -> d
                   f"({t_all:.3f} s)",
         "runs_s": runs_all,
         "one_core": {"value": round(v_one, 3), "unit": "Mpix/s", "cores": 1,
-                     "sample": f"4,096 seeded random blocks (65,536 px) of the same workload; 1 warm-up, "
+                     "sample": f"16,384 seeded random blocks (262,144 px) of the same workload; 1 warm-up, "
                                f"median of 5 ({t_one:.3f} s)", "runs_s": runs_one},
         "host": host,
         "build": "g++ -O3 -ffp-contract=off -fopenmp, baseline x86-64 (no -march)",
@@ -222,6 +222,37 @@ def bench_formats(img, reps: int = 5, launches: int = 20) -> dict:
     return out
 
 
+def count_format_ops(formats: dict, img) -> None:
+    """Adds the as-written f32 op count per block (instrumented oracle, 1024
+    seeded blocks of each kernel's own input) and the FP32 roofline fraction
+    to the encode kernels of the other-kernel leg.  The BC1-BC5 counts are on
+    the same 4096x4096 texture, BC6H 'slow' on the 2048x2048 HDR texture."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    from block_compression_amd import synth
+    from block_compression_amd.settings import BC6HSettings
+
+    W = img.shape[1]
+    ids = np.sort(np.random.default_rng(7).choice((W // 4) ** 2, 1024, replace=False))
+    counts = {f"{n}_encode_{W}": oracle_lib.count_ops_rgba8(t, b"", img, ids) / 1024.0
+              for t, n in [(0, "bc1"), (2, "bc3"), (3, "bc4"), (4, "bc5")]}
+    hdr = synth.hdr_image_rgba16f(2048, 2048, 3)
+    ids6 = np.sort(np.random.default_rng(7).choice(512 * 512, 1024, replace=False))
+    counts["bc6h_slow_encode_2048"] = oracle_lib.count_ops_rgba16(BC6HSettings.slow().to_bytes(), hdr,
+                                                                  ids6) / 1024.0
+    for key, ops in counts.items():
+        e = formats.get(key)
+        if e is None:
+            continue
+        side = 2048 if key.startswith("bc6h") else W
+        tops = ops * (side // 4) ** 2 / (e["ms"] * 1e-3) / 1e12
+        e["ops_per_block"] = round(ops, 1)
+        e["achieved_tops"] = round(tops, 3)
+        e["fp32_frac"] = round(tops / PEAK_FP32_TFLOPS, 4)
+
+
 def bench_batched(n_textures: int, reps: int, oracle_levels: int, threads: int) -> dict:
     """The batched mixed mip-chain workload (block_compression_amd.workloads):
     the same tasks encoded by per-task launches (bcn_compress, one kernel per
@@ -302,6 +333,15 @@ def bench_batched(n_textures: int, reps: int, oracle_levels: int, threads: int) 
         b2.close()
 
     batched_setup_ms = wall(batched_setup_step, reps)
+    # The other class schedule (concurrent class kernels vs one class after
+    # another) with the autotuned chunking, through its own graph.
+    other = dict(cfg, serial_classes=1 - cfg["serial_classes"])
+    workloads.queue(comp, levels, batched)
+    b3 = comp.create_batch(other)
+    b3.launch(graph=True)
+    torch.cuda.synchronize()
+    other_ms = timed(lambda: b3.launch(graph=True), reps)
+    b3.close()
     info = batch.info()
     log(f"bench: batched persistent launch {batched_ms:.2f} ms (graph replay), {batched_setup_ms:.2f} ms "
         f"with queuing and batch creation, config {cfg}")
@@ -344,6 +384,8 @@ def bench_batched(n_textures: int, reps: int, oracle_levels: int, threads: int) 
         "batched_with_setup_ms": round(batched_setup_ms, 3),
         "speedup_with_setup": round(plain_setup_ms / batched_setup_ms, 3),
         "autotuned_config": cfg,
+        "schedules_ms": {"concurrent" if cfg["serial_classes"] == 0 else "serial": round(batched_ms, 3),
+                         "serial" if cfg["serial_classes"] == 0 else "concurrent": round(other_ms, 3)},
         "grid": info["grid"],
         "chunks": info["chunks"],
         "identical_to_plain": same,
@@ -353,7 +395,9 @@ def bench_batched(n_textures: int, reps: int, oracle_levels: int, threads: int) 
                   "compress() of the already-queued tasks, one kernel per mip level as the reference "
                   "dispatches; batched = one replay of the batch's hipGraph.  With set-up (wall clock "
                   "to a device sync): plain = queue every level + compress(); batched = queue every "
-                  "level + create_batch (queue upload) + launch.",
+                  "level + create_batch (queue upload) + launch.  schedules_ms: the autotuned "
+                  "chunking with the three class kernels concurrent (three streams) and one after "
+                  "another.",
     }
 
 
@@ -638,6 +682,8 @@ def main() -> int:
     if not args.no_formats:
         log("bench: other formats (BC1/BC3/BC4/BC5 4096^2, BC6H slow 2048^2, BC7 decode)")
         formats = bench_formats(img)
+        if not args.no_cpu_baseline:
+            count_format_ops(formats, img)
 
     batched = None
     if args.batched_textures > 0:

@@ -87,7 +87,8 @@ class RefineParams(ctypes.Structure):
 class BatchConfig(ctypes.Structure):
     """bcn_batch_config."""
 
-    _fields_ = [("chunk_blocks_bc7", _u32), ("chunk_blocks_bc15", _u32), ("waves_per_cu", _u32)]
+    _fields_ = [("chunk_blocks_bc7", _u32), ("chunk_blocks_bc15", _u32), ("waves_per_cu", _u32),
+                ("serial_classes", _u32)]
 
     def as_dict(self) -> dict:
         return {k: int(getattr(self, k)) for k, _ in self._fields_}

@@ -131,8 +131,9 @@ class GpuBlockCompressor:
 
 
     def create_batch(self, config: Optional[dict] = None) -> "BlockCompressionBatch":
-        """Moves every queued task into a BlockCompressionBatch (one persistent
-        launch over all of them; see bcn_batch_create)."""
+        """Moves every queued task into a BlockCompressionBatch (one launch
+        over all of them: a persistent kernel per kernel class, the classes
+        running concurrently; see bcn_batch_create)."""
         return BlockCompressionBatch(self, config)
 
 
@@ -216,9 +217,10 @@ def _stream_handle(device, stream) -> int:
 
 
 class BlockCompressionBatch:
-    """Every task queued on a GpuBlockCompressor, encoded by one persistent
-    kernel over a block work queue (no counterpart in the reference, which
-    dispatches per task).  launch() may be called any number of times; the
+    """Every task queued on a GpuBlockCompressor, encoded by persistent
+    kernels over a block work queue, one per kernel class, running
+    concurrently (no counterpart in the reference, which dispatches per
+    task).  launch() may be called any number of times; the
     textures and buffers of the tasks must stay alive while the batch does."""
 
     def __init__(self, compressor: GpuBlockCompressor, config: Optional[dict] = None):

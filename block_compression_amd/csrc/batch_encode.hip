@@ -1,6 +1,6 @@
-// Batched encode: one persistent kernel over a device-resident block work
-// queue that spans any number of tasks (textures, mip levels) of mixed
-// formats.
+// Batched encode: persistent kernels over a device-resident block work queue
+// that spans any number of tasks (textures, mip levels) of mixed formats, one
+// kernel per kernel class, running concurrently.
 //
 // The reference records one compute dispatch per task
 // (GpuBlockCompressor::compress, src/block_compressor.rs:540-590).  Small
@@ -8,8 +8,9 @@
 // four waves on a 256-CU part, and its dispatch lasts as long as one lane
 // takes to search one block.  Here every BC1-BC5 and BC7 task of a batch is
 // cut into work items ("chunks": consecutive blocks of one task), the chunks
-// are ordered most expensive first, and a grid sized to the resident wave
-// count pulls them from an atomic counter until the queue is empty.  Each
+// are ordered most expensive first, and per kernel class a grid sized to the
+// resident wave count pulls them from an atomic counter until the class's
+// queue is empty; the classes' kernels run side by side (launch_on).  Each
 // 64-lane wave runs one chunk at a time, so the format branch is uniform
 // across the wave.  A chunk's blocks are encoded by the same block functions
 // as the per-format kernels, so the bytes are identical.
@@ -186,6 +187,11 @@ struct bcn_batch {
   uint32_t max_resident[bcn::kQueueClasses] = {1, 1, 1};  // resident workgroups per kernel
   bcn_batch_config cfg{};
   hipStream_t capture = nullptr;
+  // Fork/join of the class kernels: the launch stream runs the BC7 RGBA
+  // class, side[0] the BC7 RGB class, side[1] the BC1-BC5 class and then the
+  // BC6H tasks; default priority, non-blocking (they join graph captures).
+  hipStream_t side[2] = {nullptr, nullptr};
+  hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::string last_error;
@@ -216,7 +222,7 @@ void drop_graph(bcn_batch* b) {
 bool valid_config(const bcn_batch_config& c) {
   return c.chunk_blocks_bc7 >= 64 && c.chunk_blocks_bc7 % 64 == 0 && c.chunk_blocks_bc15 >= 64 &&
          c.chunk_blocks_bc15 % 64 == 0 && c.chunk_blocks_bc7 <= (1u << 24) &&
-         c.chunk_blocks_bc15 <= (1u << 24);
+         c.chunk_blocks_bc15 <= (1u << 24) && c.serial_classes <= 1;
 }
 
 // Cuts the queue tasks into chunks under `cfg` (grouped by class, each class
@@ -224,7 +230,7 @@ bool valid_config(const bcn_batch_config& c) {
 int build_queue(bcn_batch* b, const bcn_batch_config& cfg) {
   if (!valid_config(cfg))
     return bfail(b, BCN_ERR_INVALID_ARGUMENT,
-                 "chunk sizes must be positive multiples of 64 (at most 2^24)");
+                 "chunk sizes must be positive multiples of 64 (at most 2^24), serial_classes 0 or 1");
   std::vector<bcn::QueueChunk> chunks;
   for (int cls = 0; cls < bcn::kQueueClasses; cls++) {
     b->class_begin[cls] = (uint32_t)chunks.size();
@@ -279,19 +285,55 @@ void launch_class(bcn_batch* b, hipStream_t stream) {
                      b->d_next + CLASS);
 }
 
+// The rewind kernel, then the three class kernels forked onto three streams
+// so that they run concurrently: each is a persistent grid that drains its
+// own queue, and when one class's queue runs dry its workgroups leave and
+// the others' waiting workgroups take the freed slots, so one class's tail
+// is filled by the others' work instead of idling until the next launch.
+// The BC6H tasks follow the BC1-BC5 class on its stream.  The launch stream
+// waits for both side streams, so work queued after the launch sees every
+// block written.  Under stream capture the side streams join the capture
+// through the fork event, and the graph gets the same three branches.
+// With serial_classes every kernel goes on the launch stream, one class
+// after the other.
 int launch_on(bcn_batch* b, hipStream_t stream) {
   hipLaunchKernelGGL(bcn::queue_rewind_kernel, dim3(1), dim3(64), 0, stream, b->d_next);
-  launch_class<bcn::kQueueBc7Rgba>(b, stream);
-  launch_class<bcn::kQueueBc7Rgb>(b, stream);
-  launch_class<bcn::kQueueBc15>(b, stream);
   hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return bfail(b, BCN_ERR_HIP, "launch of the persistent kernels failed", e);
-  for (const bcn::Task& t : b->per_task) {
-    e = bcn::launch_task(t, stream);
-    if (e == hipErrorNotSupported) return bfail(b, BCN_ERR_UNSUPPORTED, "format not built in this library");
-    if (e != hipSuccess) return bfail(b, BCN_ERR_HIP, "kernel launch failed", e);
+  if (b->cfg.serial_classes) {
+    launch_class<bcn::kQueueBc7Rgba>(b, stream);
+    launch_class<bcn::kQueueBc7Rgb>(b, stream);
+    launch_class<bcn::kQueueBc15>(b, stream);
+    e = hipGetLastError();
+    if (e != hipSuccess) return bfail(b, BCN_ERR_HIP, "launch of the persistent kernels failed", e);
+    for (const bcn::Task& t : b->per_task) {
+      e = bcn::launch_task(t, stream);
+      if (e == hipErrorNotSupported) return bfail(b, BCN_ERR_UNSUPPORTED, "format not built in this library");
+      if (e != hipSuccess) return bfail(b, BCN_ERR_HIP, "kernel launch failed", e);
+    }
+    return BCN_OK;
   }
-  return BCN_OK;
+  if (e == hipSuccess) e = hipEventRecord(b->fork, stream);
+  if (e == hipSuccess) e = hipStreamWaitEvent(b->side[0], b->fork, 0);
+  if (e == hipSuccess) e = hipStreamWaitEvent(b->side[1], b->fork, 0);
+  if (e != hipSuccess) return bfail(b, BCN_ERR_HIP, "fork of the class streams failed", e);
+  launch_class<bcn::kQueueBc7Rgba>(b, stream);
+  launch_class<bcn::kQueueBc7Rgb>(b, b->side[0]);
+  launch_class<bcn::kQueueBc15>(b, b->side[1]);
+  e = hipGetLastError();
+  int st = BCN_OK;
+  if (e != hipSuccess) st = bfail(b, BCN_ERR_HIP, "launch of the persistent kernels failed", e);
+  for (size_t i = 0; st == BCN_OK && i < b->per_task.size(); i++) {
+    e = bcn::launch_task(b->per_task[i], b->side[1]);
+    if (e == hipErrorNotSupported) st = bfail(b, BCN_ERR_UNSUPPORTED, "format not built in this library");
+    else if (e != hipSuccess) st = bfail(b, BCN_ERR_HIP, "kernel launch failed", e);
+  }
+  // Join even after a failed launch, so that a capture in progress can end.
+  for (int i = 0; i < 2; i++) {
+    hipError_t je = hipEventRecord(b->join[i], b->side[i]);
+    if (je == hipSuccess) je = hipStreamWaitEvent(stream, b->join[i], 0);
+    if (je != hipSuccess && st == BCN_OK) st = bfail(b, BCN_ERR_HIP, "join of the class streams failed", je);
+  }
+  return st;
 }
 
 template <int CLASS>
@@ -314,9 +356,10 @@ int bcn_batch_create(bcn_compressor* c, const bcn_batch_config* config, bcn_batc
   cfg.chunk_blocks_bc7 = 64;
   cfg.chunk_blocks_bc15 = 1024;
   cfg.waves_per_cu = 0;
+  cfg.serial_classes = 0;
   if (config) cfg = *config;
   if (!valid_config(cfg)) {
-    c->last_error = "chunk sizes must be positive multiples of 64 (at most 2^24)";
+    c->last_error = "chunk sizes must be positive multiples of 64 (at most 2^24), serial_classes 0 or 1";
     return BCN_ERR_INVALID_ARGUMENT;
   }
   bcn::DeviceGuard guard(c->device);
@@ -380,6 +423,11 @@ int bcn_batch_create(bcn_compressor* c, const bcn_batch_config* config, bcn_batc
                   hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMalloc(&b->d_next, bcn::kQueueClasses * sizeof(uint32_t));
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking);
+  for (int i = 0; i < 2; i++) {
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming);
+  }
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
   if (e != hipSuccess) {
     char buf[256];
     snprintf(buf, sizeof(buf), "batch set-up failed: %s", hipGetErrorString(e));
@@ -482,7 +530,8 @@ int bcn_batch_autotune(bcn_batch* b, void* hip_stream, int reps, bcn_batch_confi
       best = cfg;
     }
   };
-  // Coordinate search: BC1-BC5 chunk size, then BC7 chunk size, then grid.
+  // Coordinate search: BC1-BC5 chunk size, then BC7 chunk size, then grid,
+  // then the class schedule.
   const uint32_t bc15_sizes[] = {256, 1024, 4096};
   const uint32_t bc7_sizes[] = {64, 128, 256};
   for (uint32_t s : bc15_sizes) {
@@ -501,6 +550,12 @@ int bcn_batch_autotune(bcn_batch* b, void* hip_stream, int reps, bcn_batch_confi
     if (w != 0 && w == resident_per_cu / 2 && resident_per_cu < 2) continue;
     bcn_batch_config cfg = best;
     cfg.waves_per_cu = w;
+    consider(cfg);
+  }
+  // Last, the class schedule: concurrent or one class after another.
+  {
+    bcn_batch_config cfg = best;
+    cfg.serial_classes = best.serial_classes ? 0u : 1u;
     consider(cfg);
   }
   (void)hipEventDestroy(e0);
@@ -535,6 +590,11 @@ void bcn_batch_destroy(bcn_batch* b) {
   bcn::DeviceGuard guard(b->device);
   drop_graph(b);
   if (b->capture) (void)hipStreamDestroy(b->capture);
+  for (int i = 0; i < 2; i++) {
+    if (b->side[i]) (void)hipStreamDestroy(b->side[i]);
+    if (b->join[i]) (void)hipEventDestroy(b->join[i]);
+  }
+  if (b->fork) (void)hipEventDestroy(b->fork);
   if (b->d_tasks) (void)hipFree(b->d_tasks);
   if (b->d_chunks) (void)hipFree(b->d_chunks);
   if (b->d_next) (void)hipFree(b->d_next);

@@ -260,8 +260,11 @@ void bcn_partition_tables(uint32_t* pattern, uint32_t* mask01, uint8_t* anchor1,
  * persistent kernels over a device-resident block work queue (chunks of
  * blocks of one task, most expensive first), one kernel per kernel class
  * (BC7 with alpha, BC7 opaque, BC1-BC5) so that each class keeps its own
- * register allocation; BC6H tasks of the batch are launched per task after
- * them.  The output bytes are the ones bcn_compress writes. */
+ * register allocation.  The three class kernels run concurrently on three
+ * streams (the launch stream and two of the batch's own, default priority),
+ * so one class's tail is filled by the others; BC6H tasks of the batch are
+ * launched per task after the BC1-BC5 class.  The output bytes are the ones
+ * bcn_compress writes. */
 typedef struct bcn_batch bcn_batch;
 
 typedef struct bcn_batch_config {
@@ -269,6 +272,8 @@ typedef struct bcn_batch_config {
   uint32_t chunk_blocks_bc15;  /* blocks per work item for BC1-BC5 tasks (multiple of 64) */
   uint32_t waves_per_cu;       /* persistent grid = CUs x this, one 64-lane wave per
                                   workgroup; 0 = as many as can be resident           */
+  uint32_t serial_classes;     /* 0 = the class kernels run concurrently (default);
+                                  1 = one after another on the launch stream          */
 } bcn_batch_config;
 
 typedef struct bcn_batch_info {
@@ -287,9 +292,12 @@ typedef struct bcn_batch_info {
 int bcn_batch_create(bcn_compressor* c, const bcn_batch_config* config, bcn_batch** out);
 
 /* Enqueues the batch on hip_stream: a one-thread kernel that rewinds the
- * queue, the persistent kernel, then any BC6H tasks.  Allocates nothing and
- * issues no memset, so it may be captured into a hipGraph.  A batch may be
- * launched any number of times; launches of one batch must not overlap. */
+ * queue, then the three class kernels forked onto hip_stream and the
+ * batch's two side streams (the BC6H tasks after the BC1-BC5 class), joined
+ * back into hip_stream (with serial_classes, all on hip_stream): work
+ * enqueued on hip_stream afterwards sees every block.  Allocates nothing and issues no memset, so it may be captured into
+ * a hipGraph (the side streams join the capture).  A batch may be launched
+ * any number of times; launches of one batch must not overlap. */
 int bcn_batch_launch(bcn_batch* b, void* hip_stream);
 
 /* As bcn_batch_launch through a hipGraph: the first call captures the launch

@@ -127,6 +127,7 @@ def test_batch_graph_replays_and_autotune_keep_bytes():
             assert np.array_equal(b.cpu().numpy(), r), rep
     cfg = batch.autotune(reps=1)
     assert cfg["chunk_blocks_bc7"] % 64 == 0 and cfg["chunk_blocks_bc15"] % 64 == 0
+    assert cfg["serial_classes"] in (0, 1)
     assert batch.info()["config"] == cfg
     for b in bufs:
         b.fill_(0)
@@ -134,6 +135,34 @@ def test_batch_graph_replays_and_autotune_keep_bytes():
     torch.cuda.synchronize()
     for b, r in zip(bufs, ref):
         assert np.array_equal(b.cpu().numpy(), r)
+    batch.close()
+
+
+@pytest.mark.parametrize("serial", [0, 1])
+def test_batch_class_schedules_write_the_same_bytes(serial):
+    """The class kernels concurrent (three streams, joined) or one after
+    another: the same bytes as per-task launches, eagerly and through the
+    graph, and work queued on the launch stream afterwards sees them."""
+    from block_compression_amd import GpuBlockCompressor
+
+    print(f"batch: serial_classes={serial}", flush=True)
+    texs = _setup(CASES)
+    ref = _per_task(CASES, texs)
+    bufs = _bufs(CASES)
+    c = GpuBlockCompressor()
+    _queue(c, CASES, bufs, texs)
+    batch = c.create_batch({"chunk_blocks_bc7": 64, "chunk_blocks_bc15": 256, "waves_per_cu": 0,
+                            "serial_classes": serial})
+    assert batch.info()["config"]["serial_classes"] == serial
+    for graph in (False, True, True):
+        for b in bufs:
+            b.fill_(0x5A)
+        batch.launch(graph=graph)
+        # Same-stream consumers after the join: copies enqueued right after the launch.
+        got = [b.clone() for b in bufs]
+        torch.cuda.synchronize()
+        for g, r in zip(got, ref):
+            assert np.array_equal(g.cpu().numpy(), r), graph
     batch.close()
 
 
@@ -149,6 +178,9 @@ def test_empty_batch_and_bad_config():
     batch.close()
     with pytest.raises(BcnError):
         c.create_batch({"chunk_blocks_bc7": 100, "chunk_blocks_bc15": 64, "waves_per_cu": 0})
+    with pytest.raises(BcnError):
+        c.create_batch({"chunk_blocks_bc7": 64, "chunk_blocks_bc15": 64, "waves_per_cu": 0,
+                        "serial_classes": 2})
 
 
 def test_mip_chain_workload_batched_vs_plain_and_oracle():
