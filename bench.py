@@ -465,6 +465,36 @@ def bench_grad(size: int, preset: str, reps: int) -> dict:
     def psnr(sse):
         return 10.0 * math.log10(denom / sse) if sse > 0 else 0.0
 
+    # Ablations on the same blocks: the same schedule with lr = 0 (only the
+    # index reassignment and p-bit flips act), and Adam with indices and
+    # p-bits fixed.
+    def variant(src, **kw):
+        o = torch.zeros_like(src)
+        fn = lambda: grad.refine_bc7(src, td, n, n, steps=steps, channels=4, out=o, **kw)
+        r = fn()
+        ms = timed(fn)
+        return r[2].double().sum().item(), ms
+
+    sse_lr0, ms_lr0 = variant(blocks, lr=0.0, reassign=4, pbits=4)
+    sse_fixed, ms_fixed = variant(blocks, lr=lr, reassign=0, pbits=0)
+    # The gradient pass in place of the encoder's own refinement: the preset
+    # with every refine_iterations 0, then the same 64 steps.
+    import dataclasses
+
+    s_noref = dataclasses.replace(BC7_PRESETS[preset](), refine_iterations=(0,) * 8,
+                                  refine_iterations_channel=0)
+    v0 = CompressionVariant.BC7(s_noref)
+    blocks0 = torch.zeros_like(blocks)
+
+    def plain0():
+        comp.add_compression_task(v0, tex, n, n, blocks0)
+        comp.compress()
+
+    plain0()
+    plain0_ms = timed(plain0)
+    dec_n = decode_rgba8(v0, blocks0, n, n).double()
+    sse_noref = ((dec_n - td.double() * 255.0) ** 2).sum().item()
+    sse_repl, ms_repl = variant(blocks0, lr=lr, reassign=4, pbits=4)
     # The reported errors are those of the blocks written: decode and recount.
     dec = decode_rgba8(v, refined, n, n).double()
     s_dec = ((dec - td.double() * 255.0) ** 2).sum().item()
@@ -509,6 +539,22 @@ def bench_grad(size: int, preset: str, reps: int) -> dict:
         "decoded_error_matches": bool(abs(s_dec - s1) <= 1e-6 * s1 + 1.0 and abs(s_dec0 - s0) <= 1e-6 * s0 + 1.0),
         "adam": {"steps": steps, "lr": lr, "beta1": 0.9, "beta2": 0.999, "reassign_every": 4,
                  "pbit_flips_every": 4},
+        "ablation": {
+            "lr0_reassign_pbits": {"gain_db": round(psnr(sse_lr0) - psnr(s0), 4), "refine_ms": round(ms_lr0, 3)},
+            "adam_fixed_indices": {"gain_db": round(psnr(sse_fixed) - psnr(s0), 4),
+                                   "refine_ms": round(ms_fixed, 3)},
+            "note": "gains over the plain encode: the full schedule minus lr (only index reassignment "
+                    "and p-bit flips), and Adam with indices and p-bits fixed",
+        },
+        "replace_refine_iterations": {
+            "plain_no_refine_ms": round(plain0_ms, 3),
+            "psnr_plain_no_refine_db": round(psnr(sse_noref), 4),
+            "psnr_gradient_db": round(psnr(sse_repl), 4),
+            "gain_vs_plain_db": round(psnr(sse_repl) - psnr(s0), 4),
+            "time_ratio_vs_plain": round((plain0_ms + ms_repl) / plain_ms, 3),
+            "note": f"the {preset} preset with refine_iterations 0 (least-squares refinement off), "
+                    "then the 64-step gradient pass, against the plain preset with its own refinement",
+        },
         "fd_check": {"blocks": 256, "step": h, "dtype": "f64", "max_rel_err": fd_err},
         "psnr": "10 log10(255^2 / MSE) over R, G, B, A of the decoded blocks vs the FP32 target x 255",
     }

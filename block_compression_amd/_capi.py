@@ -70,6 +70,7 @@ SIGNATURES = [
     ("bcn_decode_endpoints", _i32, [_i32, _vp, _u32, _vp, _vp]),
     ("bcn_decode_with_endpoints", _i32, [_i32, _vp, _vp, _u32, _i32, _vp, _vp]),
     ("bcn_decode_with_endpoints_backward", _i32, [_i32, _vp, _vp, _u32, _vp, _vp]),
+    ("bcn_decode_texels_backward", _i32, [_i32, _vp, _vp, _u32, _vp, _vp]),
     ("bcn_bc7_refine", _i32, [_vp, _vp, _u32, _u32, _vp, _u32, _vp, _vp, _vp, _vp, _sz, _vp]),
     ("bcn_bc7_refine_workspace_size", _sz, [_u32]),
     ("bcn_bc7_refine_noalloc", _i32, [_vp, _vp, _u32, _u32, _vp, _u32, _vp, _vp, _vp, _vp]),

@@ -781,4 +781,123 @@ BCN_HD void bc1_backward(const Bc1Form& f, const float g[16][4], float grad_x[2]
     for (int c = 0; c < 3; c++) grad_x[e][c] = acc[e][c] * (255.0f / (float)bc1_qmax(c));
 }
 
+// ---- texel-side adjoint -------------------------------------------------------
+//
+// For texture training: the decoded texels as a function of the *source*
+// texels, with the block's form held fixed and its endpoints re-fitted to the
+// source by least squares -- the fit the reference's refinement step makes
+// (opt_endpoints, src/encode/common.rs:142-206), here without quantisation.
+// For one subset and one index set the fitted values are the orthogonal
+// projection of the subset's source values onto span{1, w} (w_k: texel k's
+// interpolation weight), so the map is linear, its Jacobian is that
+// projection, and the adjoint is the same projection (it is symmetric):
+//     (H g)_k = mean(g) + (w_k - mean(w)) * S_wg / S_ww,
+//     S_wg = sum_j (w_j - mean(w)) g_j,  S_ww = sum_j (w_j - mean(w))^2,
+// over the subset's texels, and mean(g) when every w is the same (the fit is
+// then the mean).  span{1, w} does not change under w -> a w + b, so the
+// weights are used in their integer units.  Decoded channels that do not
+// depend on the source (alpha of BC7 modes 0-3 and of BC1, BC1's 3-colour
+// black) and blocks with no valid mode get zero.
+
+// r[k] = (H g)_k for the texels k of `member` (bit k); other r[k] untouched.
+// Sums in texel order.
+BCN_HD void project_span_1w(uint32_t member, const float w[16], const float g[16], float r[16]) {
+  float n = 0.0f, sw = 0.0f, sg = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    if ((member >> k) & 1u) {
+      n += 1.0f;
+      sw += w[k];
+      sg += g[k];
+    }
+  }
+  if (n == 0.0f) return;
+  const float mw = sw / n, mg = sg / n;
+  float sww = 0.0f, swg = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    if ((member >> k) & 1u) {
+      const float d = w[k] - mw;
+      sww += d * d;
+      swg += d * g[k];
+    }
+  }
+  const float beta = sww > 0.0f ? swg / sww : 0.0f;
+#pragma unroll
+  for (int k = 0; k < 16; k++)
+    if ((member >> k) & 1u) r[k] = mg + (w[k] - mw) * beta;
+}
+
+// gt[k][o] = dL/d source[k][o] for g[k][o] = dL/d decoded[k][o] (BC7).  The
+// rotation swaps a colour channel with alpha on the way in and back on the
+// way out, so output channel o is fitted from source channel o with the
+// weights of internal channel bc7_rot_src(rot, o).
+BCN_HD void bc7_texel_backward(const Bc7Form& f, const float g[16][4], float gt[16][4]) {
+#pragma unroll
+  for (int k = 0; k < 16; k++)
+#pragma unroll
+    for (int o = 0; o < 4; o++) gt[k][o] = 0.0f;
+  if (f.mode > 7) return;
+  const Bc7ModeInfo mi = kBc7Modes[f.mode];
+  const uint32_t pattern = bc7_pattern(mi, f.part);
+  float wc[16], wa[16];
+  uint32_t member[3] = {0u, 0u, 0u};
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    int a, b;
+    bc7_texel_weights(f, mi, k, a, b);
+    wc[k] = (float)a;
+    wa[k] = (float)b;
+    const int s = bc7_texel_subset(pattern, k);
+    member[0] |= (s == 0 ? 1u : 0u) << k;
+    member[1] |= (s == 1 ? 1u : 0u) << k;
+    member[2] |= (s == 2 ? 1u : 0u) << k;
+  }
+#pragma unroll
+  for (int o = 0; o < 4; o++) {
+    const int c = bc7_rot_src(f.rot, o);
+    if (bc7_stored_bits(mi, c) == 0) continue;  // alpha 255 whatever the source
+    float w[16], go[16], r[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      w[k] = c < 3 ? wc[k] : wa[k];
+      go[k] = g[k][o];
+      r[k] = 0.0f;
+    }
+#pragma unroll
+    for (int s = 0; s < 3; s++)
+      if (s < (int)mi.ns) project_span_1w(member[s], w, go, r);
+#pragma unroll
+    for (int k = 0; k < 16; k++) gt[k][o] = r[k];
+  }
+}
+
+// The same for BC1: one subset, palette weights in thirds (four-colour mode:
+// indices 0, 1, 2, 3 at 0, 3, 1, 2) or halves (three-colour: 0, 2, 1; index 3
+// is black whatever the source and is left out of the fit).
+BCN_HD void bc1_texel_backward(const Bc1Form& f, const float g[16][4], float gt[16][4]) {
+  float w[16];
+  uint32_t member = 0u;
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    const uint32_t i = (f.idx >> (2 * k)) & 3u;
+    w[k] = i == 0 ? 0.0f : (i == 1 ? (f.four ? 3.0f : 2.0f) : (i == 2 ? 1.0f : 2.0f));
+    member |= (f.four || i != 3u ? 1u : 0u) << k;
+  }
+#pragma unroll
+  for (int k = 0; k < 16; k++) gt[k][3] = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    float go[16], r[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      go[k] = g[k][c];
+      r[k] = 0.0f;
+    }
+    project_span_1w(member, w, go, r);
+#pragma unroll
+    for (int k = 0; k < 16; k++) gt[k][c] = r[k];
+  }
+}
+
 }  // namespace bcn

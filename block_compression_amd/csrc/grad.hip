@@ -106,6 +106,40 @@ __global__ __launch_bounds__(kGradGroup) void backward_kernel(const uint32_t* __
   }
 }
 
+// Texel-side adjoint (bc7_texel_backward / bc1_texel_backward): gradients
+// (n, 16, 4) in, (n, 16, 4) out, one lane per block.
+template <int FMT>
+__global__ __launch_bounds__(kGradGroup) void texel_backward_kernel(const uint32_t* __restrict__ blocks,
+                                                                     const float* __restrict__ gd, uint32_t n,
+                                                                     float* __restrict__ gs) {
+  uint32_t b = blockIdx.x * kGradGroup + threadIdx.x;
+  if (b >= n) return;
+  float g[16][4], out[16][4];
+  const float4* gi = reinterpret_cast<const float4*>(gd + (size_t)b * 64);
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    float4 v = gi[k];
+    g[k][0] = v.x;
+    g[k][1] = v.y;
+    g[k][2] = v.z;
+    g[k][3] = v.w;
+  }
+  if (FMT == BCN_BC7) {
+    uint32_t in[4] = {blocks[4 * b], blocks[4 * b + 1], blocks[4 * b + 2], blocks[4 * b + 3]};
+    Bc7Form f;
+    bc7_parse(in, f);
+    bc7_texel_backward(f, g, out);
+  } else {
+    uint32_t in[2] = {blocks[2 * b], blocks[2 * b + 1]};
+    Bc1Form f;
+    bc1_parse(in, f);
+    bc1_texel_backward(f, g, out);
+  }
+  float4* o = reinterpret_cast<float4*>(gs + (size_t)b * 64);
+#pragma unroll
+  for (int k = 0; k < 16; k++) o[k] = make_float4(out[k][0], out[k][1], out[k][2], out[k][3]);
+}
+
 // ---- refinement: blocks sorted by mode, four lanes per block -------------------
 //
 // The lanes of a quad refine one block together, lane c owning internal
@@ -609,6 +643,25 @@ int bcn_decode_with_endpoints_backward(bcn_variant variant, const void* blocks, 
     hipLaunchKernelGGL(bcn::backward_kernel<BCN_BC7>, grid, block, 0, s, bp, grad_texels, nblocks, grad_x);
   else
     hipLaunchKernelGGL(bcn::backward_kernel<BCN_BC1>, grid, block, 0, s, bp, grad_texels, nblocks, grad_x);
+  return check_launch();
+}
+
+int bcn_decode_texels_backward(bcn_variant variant, const void* blocks, const float* grad_texels,
+                               uint32_t nblocks, float* grad_source, void* hip_stream) {
+  if (!fmt_ok(variant) || (nblocks && (!blocks || !grad_texels || !grad_source)))
+    return BCN_ERR_INVALID_ARGUMENT;
+  if (((uintptr_t)blocks & 3u) || ((uintptr_t)grad_texels & 15u) || ((uintptr_t)grad_source & 15u))
+    return BCN_ERR_MISALIGNED;
+  if (nblocks == 0) return BCN_OK;
+  hipStream_t s = static_cast<hipStream_t>(hip_stream);
+  dim3 grid((nblocks + bcn::kGradGroup - 1) / bcn::kGradGroup), block(bcn::kGradGroup);
+  const uint32_t* bp = static_cast<const uint32_t*>(blocks);
+  if (variant == BCN_BC7)
+    hipLaunchKernelGGL(bcn::texel_backward_kernel<BCN_BC7>, grid, block, 0, s, bp, grad_texels, nblocks,
+                       grad_source);
+  else
+    hipLaunchKernelGGL(bcn::texel_backward_kernel<BCN_BC1>, grid, block, 0, s, bp, grad_texels, nblocks,
+                       grad_source);
   return check_launch();
 }
 

@@ -9,6 +9,11 @@ parameter layout and the straight-through / smooth forwards.
     texels = DecodeWithEndpoints.apply(x, blocks, variant_tag, smooth)   # (n, 16, 4)
     loss(texels).backward()                           # x.grad via the HIP adjoint
 
+    # texture training: gradients to the source texels through the blocks
+    src = image_to_blocks(texture)                    # (n, 16, 4), 0..255 units
+    decoded = DecodeTexels.apply(src, blocks, variant_tag)
+    loss(decoded).backward()                          # texture.grad via the texel-side adjoint
+
     refined, sse0, sse1 = refine_bc7(blocks, target, width, height)
 
 torch supplies device memory and streams; the arithmetic is the HIP kernels'.
@@ -84,6 +89,78 @@ def decode_with_endpoints_backward(tag: int, blocks, grad_texels):
     _capi.check(_capi.lib().bcn_decode_with_endpoints_backward(tag, blocks.data_ptr(), g.data_ptr(), n,
                                                                 gx.data_ptr(), _stream(blocks.device)))
     return gx
+
+
+def decode_texels_backward(tag: int, blocks, grad_texels):
+    """Texel-side adjoint (bcn_decode_texels_backward): dL/d source texels
+    (n, 16, 4) for dL/d decoded texels (n, 16, 4), the decode viewed as the
+    least-squares re-fit of the source with each block's form held fixed."""
+    import torch
+
+    n = _check_blocks(blocks, tag)
+    _nparams(tag)
+    g = grad_texels.contiguous().to(torch.float32)
+    if g.shape != (n, 16, 4) or g.device != blocks.device:
+        raise ValueError(f"grad_texels must have shape ({n}, 16, 4) on the blocks' device")
+    gs = torch.empty((n, 16, 4), dtype=torch.float32, device=blocks.device)
+    _capi.check(_capi.lib().bcn_decode_texels_backward(tag, blocks.data_ptr(), g.data_ptr(), n,
+                                                        gs.data_ptr(), _stream(blocks.device)))
+    return gs
+
+
+def image_to_blocks(img):
+    """(H, W, C) -> (H/4 * W/4, 16, C): each block's texels row by row, blocks
+    in row-major order (the layout of the adjoint functions).  A view-based
+    torch reshape, so it is differentiable."""
+    h, w, c = img.shape
+    return img.reshape(h // 4, 4, w // 4, 4, c).permute(0, 2, 1, 3, 4).reshape(-1, 16, c)
+
+
+def blocks_to_image(x, height: int, width: int):
+    """Inverse of image_to_blocks."""
+    c = x.shape[-1]
+    return x.reshape(height // 4, width // 4, 4, 4, c).permute(0, 2, 1, 3, 4).reshape(height, width, c)
+
+
+class DecodeTexels:
+    """torch.autograd.Function for texture training: the decoded texels
+    (n, 16, 4) of `blocks`, the encoding of the source texels `texels`
+    (n, 16, 4).  The value is the exact decode of `blocks` (0..255 units);
+    the backward is the texel-side adjoint (decode_texels_backward), so the
+    gradient reaches the source texture through the block structure instead
+    of as the identity.  Re-encode the source (GpuBlockCompressor) whenever
+    it has changed enough for the blocks' form to change."""
+
+    @staticmethod
+    def apply(texels, blocks, tag: int):
+        return _texels_function().apply(texels, blocks, tag)
+
+
+_FN_TEXELS = None
+
+
+def _texels_function():
+    global _FN_TEXELS
+    if _FN_TEXELS is None:
+        import torch
+
+        class _DecodeTexels(torch.autograd.Function):
+            @staticmethod
+            def forward(ctx, texels, blocks, tag):
+                n = _check_blocks(blocks, tag)
+                if tuple(texels.shape) != (n, 16, 4):
+                    raise ValueError(f"texels must have shape ({n}, 16, 4)")
+                ctx.save_for_backward(blocks)
+                ctx.tag = tag
+                return decode_with_endpoints(tag, blocks, endpoints(tag, blocks), False)
+
+            @staticmethod
+            def backward(ctx, grad_out):
+                (blocks,) = ctx.saved_tensors
+                return decode_texels_backward(ctx.tag, blocks, grad_out), None, None
+
+        _FN_TEXELS = _DecodeTexels
+    return _FN_TEXELS
 
 
 class DecodeWithEndpoints:

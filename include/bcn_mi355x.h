@@ -341,6 +341,19 @@ int bcn_decode_with_endpoints_backward(bcn_variant variant, const void* blocks,
                                        const float* grad_texels, uint32_t nblocks, float* grad_x,
                                        void* hip_stream);
 
+/* Texel-side adjoint, for texture training (no counterpart in the reference):
+ * the gradient with respect to the *source* texels of the blocks' decode,
+ * viewing each block as its form (as above) with endpoints re-fitted to the
+ * source texels by least squares, unquantised (the fit of the reference's
+ * refinement step, opt_endpoints, src/encode/common.rs:142-206).  Per subset
+ * and index set that map is the orthogonal projection of the source values
+ * onto span{1, interpolation weight}, which is its own adjoint; channels that
+ * do not depend on the source (alpha of BC7 modes 0-3 and of BC1, BC1's
+ * 3-colour black texels) get 0.  grad_texels in, grad_source out: float
+ * [16][4] per block, 16-byte aligned device pointers. */
+int bcn_decode_texels_backward(bcn_variant variant, const void* blocks, const float* grad_texels,
+                               uint32_t nblocks, float* grad_source, void* hip_stream);
+
 typedef struct bcn_refine_params {
   uint32_t steps;      /* Adam steps                                           */
   float lr;            /* step size, in quantised endpoint units               */

@@ -208,3 +208,74 @@ def random_bc7_blocks(n: int, seed: int) -> np.ndarray:
         m = i % 8
         blocks[i, 0] = (blocks[i, 0] & (0xFF & ~((1 << (m + 1)) - 1))) | (1 << m)
     return blocks
+
+
+# ---- texel-side adjoint ------------------------------------------------------
+#
+# The decode as a function of the source texels: each subset's endpoints
+# re-fitted to the source by least squares (one index set at a time), then
+# the relaxed decode.  Its Jacobian per subset and channel is the hat matrix
+# A pinv(A) of the design A = [1 - w, w] (w: the texels' weights in [0, 1]);
+# computed here with numpy's pinv in float64, independently of the kernels'
+# closed form.
+
+
+def _hat(w: np.ndarray) -> np.ndarray:
+    a = np.stack([1.0 - w, w], axis=1)
+    return a @ np.linalg.pinv(a)
+
+
+def texel_adjoint_bc7(f, g: np.ndarray) -> np.ndarray:
+    """dL/d source (16, 4) for dL/d decoded g (16, 4), float64."""
+    out = np.zeros((16, 4))
+    if f["mode"] == 8:
+        return out
+    wc, wa = _weights(f)
+    for c in range(4):
+        if _bits(f, c) == 0:
+            continue  # alpha decodes as 255 whatever the source
+        o = _out_channel(f["rot"], c)
+        w = (wc if c < 3 else wa) / 64.0
+        for s in range(f["ns"]):
+            m = f["subset"] == s
+            if m.any():
+                out[m, o] = _hat(w[m]).T @ g[m, o]
+    return out
+
+
+def texel_adjoint_bc1(f, g: np.ndarray) -> np.ndarray:
+    out = np.zeros((16, 4))
+    if f["four"]:
+        wt = np.array([0.0, 1.0, 1.0 / 3.0, 2.0 / 3.0])
+        m = np.ones(16, bool)
+    else:
+        wt = np.array([0.0, 1.0, 0.5, 0.0])
+        m = f["idx"] != 3  # black whatever the source
+    w = wt[f["idx"]]
+    if m.any():
+        h = _hat(w[m])
+        for c in range(3):
+            out[m, c] = h.T @ g[m, c]
+    return out
+
+
+def refit_decode_bc7(f, t: np.ndarray) -> np.ndarray:
+    """The map the texel adjoint differentiates, float64: per subset and
+    channel the least-squares endpoints of the source t (16, 4), then the
+    relaxed interpolation; channels without stored bits decode to 255."""
+    out = np.full((16, 4), 255.0)
+    if f["mode"] == 8:
+        return np.zeros((16, 4))
+    wc, wa = _weights(f)
+    for c in range(4):
+        o = _out_channel(f["rot"], c)
+        if _bits(f, c) == 0:
+            continue
+        w = (wc if c < 3 else wa) / 64.0
+        for s in range(f["ns"]):
+            m = f["subset"] == s
+            if m.any():
+                a = np.stack([1.0 - w[m], w[m]], axis=1)
+                e = np.linalg.lstsq(a, t[m, o], rcond=None)[0]
+                out[m, o] = a @ e
+    return out

@@ -237,6 +237,25 @@ extern "C" void kcheck_backward(int fmt, const uint32_t* blocks, const float* g,
 }
 
 // target: float[n][16][4] in 0..255 units (block-major).
+extern "C" void kcheck_texel_backward(int fmt, const uint32_t* blocks, const float* g, size_t n, float* gs) {
+  for (size_t b = 0; b < n; b++) {
+    float gv[16][4], o[16][4];
+    for (int k = 0; k < 16; k++)
+      for (int c = 0; c < 4; c++) gv[k][c] = g[b * 64 + k * 4 + c];
+    if (fmt == 6) {
+      bcn::Bc7Form f;
+      bcn::bc7_parse(blocks + 4 * b, f);
+      bcn::bc7_texel_backward(f, gv, o);
+    } else {
+      bcn::Bc1Form f;
+      bcn::bc1_parse(blocks + 2 * b, f);
+      bcn::bc1_texel_backward(f, gv, o);
+    }
+    for (int k = 0; k < 16; k++)
+      for (int c = 0; c < 4; c++) gs[b * 64 + k * 4 + c] = o[k][c];
+  }
+}
+
 extern "C" void kcheck_bc7_refine(const uint32_t* blocks, const float* target, size_t n, uint32_t steps,
                                   float lr, uint32_t channels, uint32_t reassign, uint32_t* out, float* sse0,
                                   float* sse1) {

@@ -120,6 +120,15 @@ def backward(fmt: int, blocks: np.ndarray, g: np.ndarray) -> np.ndarray:
     return gx
 
 
+def texel_backward(fmt: int, blocks: np.ndarray, g: np.ndarray) -> np.ndarray:
+    """Host build of the texel-side adjoint: g (n, 16, 4) -> (n, 16, 4)."""
+    blocks = np.ascontiguousarray(blocks, dtype=np.uint8)
+    g = np.ascontiguousarray(g, dtype=np.float32)
+    out = np.zeros_like(g)
+    lib().kcheck_texel_backward(fmt, _p(blocks), _p(g), g.shape[0], _p(out))
+    return out
+
+
 def bc7_refine(blocks: np.ndarray, target: np.ndarray, steps: int, lr: float, channels: int,
                reassign: int = 0):
     blocks = np.ascontiguousarray(blocks, dtype=np.uint8)

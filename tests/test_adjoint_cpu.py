@@ -164,3 +164,67 @@ def test_refine_never_worse_and_reports_exact_error():
     # Blocks it leaves alone stay byte-identical.
     same = s1 == s0
     assert np.array_equal(out[same], blocks[same])
+
+
+# ---- texel-side adjoint (bc7_texel_backward / bc1_texel_backward) -----------
+
+
+def _bc1_blocks(n, seed):
+    return np.random.default_rng(seed).integers(0, 256, (n, 8), dtype=np.uint8)
+
+
+@pytest.mark.parametrize("fmt", [6, 0], ids=["bc7", "bc1"])
+def test_texel_adjoint_host_build_matches_fp64_hat_matrix(fmt):
+    """The kernels' closed form (host build) equals A pinv(A) from numpy in
+    float64, per subset and index set, on random blocks of every BC7 mode
+    (and blocks with no valid mode) and of both BC1 colour modes."""
+    import kernel_check_lib as kc
+
+    b = ref.random_bc7_blocks(800, 41) if fmt == 6 else _bc1_blocks(800, 41)
+    if fmt == 6:
+        b[-8:, 0] = 0  # no valid mode
+    g = np.random.default_rng(42).standard_normal((len(b), 16, 4)).astype(np.float32)
+    got = kc.texel_backward(fmt, b.reshape(-1), g)
+    for i in range(len(b)):
+        if fmt == 6:
+            want = ref.texel_adjoint_bc7(ref.parse_bc7(b[i].tobytes()), g[i].astype(np.float64))
+        else:
+            want = ref.texel_adjoint_bc1(ref.parse_bc1(b[i].tobytes()), g[i].astype(np.float64))
+        np.testing.assert_allclose(got[i], want, rtol=1e-5, atol=2e-5, err_msg=f"block {i}")
+
+
+def test_texel_adjoint_fp64_reference_is_the_refit_jacobian():
+    """The reference hat matrix is the Jacobian of the re-fit decode: central
+    differences of refit_decode_bc7 (a linear map of the source) reproduce
+    the adjoint's action, <J^T g, e_j> = <g, J e_j>."""
+    b = ref.random_bc7_blocks(64, 43)
+    rng = np.random.default_rng(44)
+    for i in range(len(b)):
+        f = ref.parse_bc7(b[i].tobytes())
+        t = rng.uniform(0, 255, (16, 4))
+        g = rng.standard_normal((16, 4))
+        adj = ref.texel_adjoint_bc7(f, g)
+        for k, o in [(rng.integers(16), rng.integers(4)) for _ in range(6)]:
+            e = np.zeros((16, 4))
+            e[k, o] = 1.0
+            h = 1e-3
+            jd = (ref.refit_decode_bc7(f, t + h * e) - ref.refit_decode_bc7(f, t - h * e)) / (2 * h)
+            assert abs(np.sum(g * jd) - adj[k, o]) < 1e-6 * (1 + abs(adj[k, o])), (i, k, o)
+
+
+@pytest.mark.parametrize("fmt", [6, 0], ids=["bc7", "bc1"])
+def test_texel_adjoint_is_a_symmetric_projection(fmt):
+    """H H g = H g and <H g1, g2> = <g1, H g2> on the host build."""
+    import kernel_check_lib as kc
+
+    b = ref.random_bc7_blocks(512, 45) if fmt == 6 else _bc1_blocks(512, 45)
+    rng = np.random.default_rng(46)
+    g1 = rng.standard_normal((len(b), 16, 4)).astype(np.float32)
+    g2 = rng.standard_normal((len(b), 16, 4)).astype(np.float32)
+    h1 = kc.texel_backward(fmt, b.reshape(-1), g1)
+    hh1 = kc.texel_backward(fmt, b.reshape(-1), h1)
+    np.testing.assert_allclose(hh1, h1, rtol=1e-4, atol=1e-4)
+    h2 = kc.texel_backward(fmt, b.reshape(-1), g2)
+    lhs = np.sum(h1.astype(np.float64) * g2, axis=(1, 2))
+    rhs = np.sum(g1.astype(np.float64) * h2, axis=(1, 2))
+    np.testing.assert_allclose(lhs, rhs, rtol=1e-4, atol=1e-4)

@@ -121,3 +121,58 @@ def test_refine_matches_host_build_and_never_worse():
     grad.refine_bc7(inplace, td, n, n, steps=64, lr=0.5, channels=4, reassign=1, pbits=True, out=inplace)
     torch.cuda.synchronize()
     assert torch.equal(inplace, out)
+
+
+@pytest.mark.parametrize("fmt", [6, 0], ids=["bc7", "bc1"])
+def test_texel_adjoint_gpu_equals_host_build_and_fp64(fmt):
+    """bcn_decode_texels_backward == the host build of the same code (bit for
+    bit) on 4096 random blocks of every mode, and == the float64 hat matrix
+    (numpy pinv) on 256 of them."""
+    import kernel_check_lib as kcl
+    from block_compression_amd import grad
+
+    print(f"grad: texel adjoint fmt {fmt}", flush=True)
+    b = _blocks(fmt, 4096, 31)
+    bd = torch.from_numpy(b.reshape(-1).copy()).cuda()
+    g = np.random.default_rng(32).standard_normal((len(b), 16, 4)).astype(np.float32)
+    got = grad.decode_texels_backward(fmt, bd, torch.from_numpy(g).cuda()).cpu().numpy()
+    assert np.array_equal(got, kcl.texel_backward(fmt, b.reshape(-1), g))
+    for i in range(256):
+        if fmt == 6:
+            want = ref.texel_adjoint_bc7(ref.parse_bc7(b[i].tobytes()), g[i].astype(np.float64))
+        else:
+            want = ref.texel_adjoint_bc1(ref.parse_bc1(b[i].tobytes()), g[i].astype(np.float64))
+        np.testing.assert_allclose(got[i], want, rtol=1e-5, atol=2e-5)
+
+
+def test_decode_texels_autograd_on_an_encoded_texture():
+    """Texture training through DecodeTexels: the value is the exact decode
+    of the GPU encoding of the texture (== the oracle decoder), and the
+    gradient reaching the (H, W, 4) texture through image_to_blocks is the
+    texel-side adjoint of the loss gradient, per block == float64."""
+    from block_compression_amd import GpuBlockCompressor, grad, synth
+    from block_compression_amd.settings import BC7Settings, CompressionVariant
+
+    print("grad: DecodeTexels on a 64x64 BC7 texture", flush=True)
+    n = 64
+    img = synth.tile_image_rgba8(n, n, 47, 0.2)
+    v = CompressionVariant.BC7(BC7Settings.alpha_basic())
+    blocks = torch.zeros(v.blocks_byte_size(n, n), dtype=torch.uint8, device="cuda")
+    c = GpuBlockCompressor()
+    c.add_compression_task(v, torch.from_numpy(img).cuda(), n, n, blocks)
+    c.compress()
+    tex = torch.tensor(img, dtype=torch.float32, device="cuda", requires_grad=True)
+    src = grad.image_to_blocks(tex)
+    assert torch.equal(grad.blocks_to_image(src, n, n), tex)
+    dec = grad.DecodeTexels.apply(src, blocks, 6)
+    b = blocks.cpu().numpy().reshape(-1, 16)
+    want_dec = oracle_lib.decompress_rgba8(6, n, n, blocks.cpu().numpy())
+    assert np.array_equal(grad.blocks_to_image(dec, n, n).detach().cpu().numpy(), want_dec.astype(np.float32))
+    target = torch.from_numpy(synth.tile_image_rgba8(n, n, 48, 0.0).astype(np.float32)).cuda()
+    loss = ((dec - grad.image_to_blocks(target)) ** 2).sum()
+    loss.backward()
+    gsrc = grad.image_to_blocks(tex.grad).cpu().numpy()
+    gdec = (2.0 * (dec - grad.image_to_blocks(target))).detach().cpu().numpy().astype(np.float64)
+    for i in range(len(b)):
+        want = ref.texel_adjoint_bc7(ref.parse_bc7(b[i].tobytes()), gdec[i])
+        np.testing.assert_allclose(gsrc[i], want, rtol=1e-4, atol=1e-3 * (1 + np.abs(gdec[i]).max()))
