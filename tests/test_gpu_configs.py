@@ -1,4 +1,15 @@
-"""The two extension workloads at their full sizes on the GPU.
+"""The benchmark configurations at their full sizes on the GPU.
+
+C1: BC1 512x512 (tile generator seed 1, no noise tiles): every block equals
+the oracle, the GPU decode equals the oracle's decoder, and the reference
+harness's PSNR (tests/metrics.rs:40-114) of the two is the same.
+
+C2 (BC4 from R, BC5 from RG, 4096x4096, seed 2 with 10% noise tiles): every
+block equals the oracle.  (BC7 opaque_basic on the same texture is checked
+block by block in test_gpu_bc7.py.)
+
+C3: BC6H slow on the 2048x2048 HDR texture: every one of the 262,144 blocks
+equals the oracle.
 
 C4: 300 tile-generator textures with full box-filtered mip chains in mixed
 formats (block_compression_amd.workloads, 3,513 levels, about 1.7 Gpx): the
@@ -22,6 +33,55 @@ from block_compression_amd.settings import BC7Settings, CompressionVariant
 torch = pytest.importorskip("torch")
 
 pytestmark = pytest.mark.gpu
+
+
+def _encode(variant, tex, w, h):
+    buf = torch.zeros(variant.blocks_byte_size(w, h), dtype=torch.uint8, device="cuda")
+    c = GpuBlockCompressor()
+    c.add_compression_task(variant, tex, w, h, buf)
+    c.compress()
+    torch.cuda.synchronize()
+    return buf
+
+
+def test_c1_bc1_512_every_block_decode_and_psnr():
+    from block_compression_amd import decode_rgba8
+    from metrics_ref import calculate_image_metrics_rgba8
+
+    print("configs: C1 BC1 512^2", flush=True)
+    img = synth.tile_image_rgba8(512, 512, 1, 0.0)
+    v = CompressionVariant.BC1()
+    buf = _encode(v, torch.from_numpy(img).cuda(), 512, 512)
+    got = buf.cpu().numpy()
+    ref = oracle_lib.compress_rgba8(v.tag, v.settings_bytes(), img, threads=8)
+    assert np.array_equal(got, ref)
+    dec = decode_rgba8(v, buf, 512, 512).cpu().numpy()
+    want = oracle_lib.decompress_rgba8(v.tag, 512, 512, ref)
+    assert np.array_equal(dec, want)
+    p_gpu = calculate_image_metrics_rgba8(img, dec, 3)["overall_psnr"]
+    p_ref = calculate_image_metrics_rgba8(img, want, 3)["overall_psnr"]
+    assert p_gpu == p_ref and p_ref > 25.0
+
+
+@pytest.mark.parametrize("fmt", ["bc4", "bc5"])
+def test_c2_bc4_bc5_4096_every_block(fmt):
+    print(f"configs: C2 {fmt} 4096^2", flush=True)
+    img = synth.tile_image_rgba8(4096, 4096, 2, 0.1)
+    v = CompressionVariant.BC4() if fmt == "bc4" else CompressionVariant.BC5()
+    got = _encode(v, torch.from_numpy(img).cuda(), 4096, 4096).cpu().numpy()
+    assert np.array_equal(got, oracle_lib.compress_rgba8(v.tag, v.settings_bytes(), img, threads=16))
+
+
+def test_c3_bc6h_slow_2048_every_block():
+    from block_compression_amd.settings import BC6HSettings
+
+    print("configs: C3 BC6H slow 2048^2 (oracle on every block)", flush=True)
+    img = synth.hdr_image_rgba16f(2048, 2048, 3)
+    s = BC6HSettings.slow()
+    v = CompressionVariant.BC6H(s)
+    tex = torch.from_numpy(img.view(np.uint16)).cuda().view(torch.float16)
+    got = _encode(v, tex, 2048, 2048).cpu().numpy()
+    assert np.array_equal(got, oracle_lib.compress_rgba16(s.to_bytes(), img, threads=16))
 
 
 def test_c4_batched_mip_chains_full_size():
