@@ -148,8 +148,8 @@ def _texels_function():
             @staticmethod
             def forward(ctx, texels, blocks, tag):
                 n = _check_blocks(blocks, tag)
-                if tuple(texels.shape) != (n, 16, 4):
-                    raise ValueError(f"texels must have shape ({n}, 16, 4)")
+                if tuple(texels.shape) != (n, 16, 4) or texels.device != blocks.device:
+                    raise ValueError(f"texels must have shape ({n}, 16, 4), on the blocks' device")
                 ctx.save_for_backward(blocks)
                 ctx.tag = tag
                 return decode_with_endpoints(tag, blocks, endpoints(tag, blocks), False)

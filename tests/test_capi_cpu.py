@@ -83,3 +83,23 @@ def test_noalloc_refine_size_and_argument_checks_without_gpu():
                                       None, None, None) == 1
     assert lib.bcn_bc7_refine_noalloc(None, None, 6, 4, None, 96, ctypes.byref(p), None, None, None) == 1
     assert lib.bcn_bc7_refine_noalloc(None, None, 8, 8, None, 128, None, None, None, None) == 1
+
+
+def test_texel_adjoint_argument_checks_without_gpu():
+    """bcn_decode_texels_backward: BC1 and BC7 only, null or misaligned
+    pointers rejected, zero blocks a no-op -- all before any device work; the
+    image <-> block layout reshapes are exact inverses."""
+    import torch
+
+    from block_compression_amd import grad
+
+    lib = _capi.lib()
+    assert lib.bcn_decode_texels_backward(3, None, None, 0, None, None) == 1  # BC4: no adjoint
+    assert lib.bcn_decode_texels_backward(6, None, None, 4, None, None) == 1
+    assert lib.bcn_decode_texels_backward(6, 16, 8, 4, 16, None) == 6  # grad not 16-byte aligned
+    assert lib.bcn_decode_texels_backward(0, 16, 16, 0, 16, None) == 0
+    img = torch.arange(8 * 12 * 4, dtype=torch.float32).reshape(8, 12, 4)
+    blocks = grad.image_to_blocks(img)
+    assert blocks.shape == (6, 16, 4)
+    assert torch.equal(blocks[1, 5], img[1, 5])  # block (0, 1), texel (1, 1)
+    assert torch.equal(grad.blocks_to_image(blocks, 8, 12), img)
