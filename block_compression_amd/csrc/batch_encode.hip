@@ -60,7 +60,7 @@ __global__ void queue_rewind_kernel(uint32_t* next) {
 
 template <int CLASS>
 __device__ __forceinline__ void queue_encode_chunk(const QueueTask* t, const QueueChunk& ch,
-                                                   uint32_t lane, uint16_t* keys, uint32_t* sdata) {
+                                                   uint32_t lane, uint32_t* keys, uint32_t* sdata) {
   const uint32_t variant = t->variant, bw = t->bw, pitch = t->pitch;
   const bool vec = t->vec != 0;
   const uint8_t* tex = t->texture;
@@ -125,7 +125,7 @@ __attribute__((amdgpu_waves_per_eu(CLASS == kQueueBc15 ? 8 : BCN_BC7_WAVES,
                                    CLASS == kQueueBc15 ? 8 : BCN_BC7_WAVES))) void queue_encode_kernel(
     const QueueTask* __restrict__ tasks, const QueueChunk* __restrict__ chunks, uint32_t nchunks,
     uint32_t* __restrict__ next) {
-  __shared__ uint16_t keys[CLASS == kQueueBc15 ? 1 : 64 * kQueueWave];
+  __shared__ uint32_t keys[CLASS == kQueueBc15 ? 1 : 32 * kQueueWave];
   __shared__ uint32_t sdata[CLASS == kQueueBc15 ? 1 : 18 * kQueueWave];
   const uint32_t lane = threadIdx.x;
   for (;;) {

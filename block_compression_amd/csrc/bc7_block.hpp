@@ -200,24 +200,33 @@ BCN_HD Moments moments_cm(const uint32_t cm[4][4], uint32_t mask) {
 // x / n is div_rcp with rn = 1.0f / n.  POW2: n is a power of two, rn exact,
 // and x / n is the exact product x * rn.
 template <int CH, bool POW2 = false>
-BCN_HD void covar_from_moments(float c[10], const int s[4], const int q[10], float s14, float rn) {
-  const float s10 = (float)s[0], s11 = (float)s[1], s12 = (float)s[2];
+BCN_HD void covar_from_fmoments(float c[10], const float s[4], const float q[10], float s14, float rn) {
+  const float s10 = s[0], s11 = s[1], s12 = s[2];
   auto quo = [&](float x) { return POW2 ? x * rn : div_rcp(x, s14, rn); };
-  c[0] = (float)q[0] - quo(s10 * s10);
-  c[1] = (float)q[1] - quo(s10 * s11);
-  c[2] = (float)q[2] - quo(s10 * s12);
-  c[4] = (float)q[4] - quo(s11 * s11);
-  c[5] = (float)q[5] - quo(s11 * s12);
-  c[7] = (float)q[7] - quo(s12 * s12);
+  c[0] = q[0] - quo(s10 * s10);
+  c[1] = q[1] - quo(s10 * s11);
+  c[2] = q[2] - quo(s10 * s12);
+  c[4] = q[4] - quo(s11 * s11);
+  c[5] = q[5] - quo(s11 * s12);
+  c[7] = q[7] - quo(s12 * s12);
   if (CH == 4) {
-    float s13 = (float)s[3];
-    c[3] = (float)q[3] - quo(s10 * s13);
-    c[6] = (float)q[6] - quo(s11 * s13);
-    c[8] = (float)q[8] - quo(s12 * s13);
-    c[9] = (float)q[9] - quo(s13 * s13);
+    float s13 = s[3];
+    c[3] = q[3] - quo(s10 * s13);
+    c[6] = q[6] - quo(s11 * s13);
+    c[8] = q[8] - quo(s12 * s13);
+    c[9] = q[9] - quo(s13 * s13);
   } else {
     c[3] = c[6] = c[8] = c[9] = 0.0f;
   }
+}
+template <int CH, bool POW2 = false>
+BCN_HD void covar_from_moments(float c[10], const int s[4], const int q[10], float s14, float rn) {
+  float fs[4], fq[10];
+#pragma unroll
+  for (int i = 0; i < 4; i++) fs[i] = (float)s[i];
+#pragma unroll
+  for (int i = 0; i < 10; i++) fq[i] = (float)q[i];
+  covar_from_fmoments<CH, POW2>(c, fs, fq, s14, rn);
 }
 
 // Texel counts of the two subsets of each 2-subset shape as floats, with
@@ -1037,8 +1046,8 @@ constexpr SubsetPlan make_subset_plan(int count) {
 constexpr SubsetPlan kPlanMode0Host = make_subset_plan(16);
 constexpr SubsetPlan kPlanMode2Host = make_subset_plan(64);
 // 6 and 21 slots: 12 and 42 of the row's 64 entries.
-static_assert(kPlanMode0Host.slots * 2 <= 64 && kPlanMode2Host.slots * 2 <= 64,
-              "kept subset fits must fit the 64-entry partition-key row");
+static_assert(kPlanMode0Host.slots <= 32 && kPlanMode2Host.slots <= 32,
+              "kept subset fits must fit the 32-word partition-key row");
 BCN_TABLE SubsetPlan kPlanMode0 = make_subset_plan(16);
 BCN_TABLE SubsetPlan kPlanMode2 = make_subset_plan(64);
 
@@ -1053,7 +1062,7 @@ BCN_HD uint32_t code15_bytes(uint32_t c) {
 // fit_partition for shape 64 + part of mode 0 or 2 following the plan; both
 // modes dequantise their 5-bit codes with unpack_to_byte(q, 5).
 template <int MODE>
-BCN_HD void fit_partition_planned(const uint32_t px[16], int part, uint16_t* row, int stride,
+BCN_HD void fit_partition_planned(const uint32_t px[16], int part, uint32_t* row, int stride,
                                   SubsetRow srow, PartFit& f) {
   constexpr int BITS = MODE == 0 ? 3 : 2;
   const int part_id = part + 64;
@@ -1065,7 +1074,8 @@ BCN_HD void fit_partition_planned(const uint32_t px[16], int part, uint16_t* row
     const int op = MODE == 0 ? kPlanMode0.op[part][j] : kPlanMode2.op[part][j];
     const int slot = MODE == 0 ? kPlanMode0.slot[part][j] : kPlanMode2.slot[part][j];
     if (op == kReuse) {
-      const uint32_t c0 = row[(2 * slot) * stride], c1 = row[(2 * slot + 1) * stride];
+      // A kept fit: both endpoints' 15-bit codes in one word of the row.
+      const uint32_t c01 = row[slot * stride], c0 = c01 & 0xFFFFu, c1 = c01 >> 16;
       f.q[j][0] = code15_bytes(c0);
       f.q[j][1] = code15_bytes(c1);
 #pragma unroll
@@ -1089,8 +1099,7 @@ BCN_HD void fit_partition_planned(const uint32_t px[16], int part, uint16_t* row
       f.q[j][0] = pack4(q0, 3);
       f.q[j][1] = pack4(q1, 3);
       if (op == kFitKeep) {
-        row[(2 * slot) * stride] = (uint16_t)code15(q0);
-        row[(2 * slot + 1) * stride] = (uint16_t)code15(q1);
+        row[slot * stride] = code15(q0) | (code15(q1) << 16);
       }
     }
   }
@@ -1101,7 +1110,7 @@ BCN_HD void fit_partition_planned(const uint32_t px[16], int part, uint16_t* row
 // order (count is 16 for mode 0 and 64 for mode 2).
 template <int MODE>
 BCN_HD void enc_mode02(const uint32_t px[16], const Bc7Params& s, int count, int opaque_err,
-                       int order, uint16_t* row, int stride, SubsetRow srow, Bc7Best& best) {
+                       int order, uint32_t* row, int stride, SubsetRow srow, Bc7Best& best) {
   PartFit bf;
   bf.err = 0x7FFFFFFF;
   int best_part = 0;
@@ -1117,42 +1126,71 @@ BCN_HD void enc_mode02(const uint32_t px[16], const Bc7Params& s, int count, int
   finish_mode01237<MODE>(px, s, bf, best_part, opaque_err, order, best, srow);
 }
 
+// The lane's 64 partition keys in 32 words (word w at p[w * stride]): the
+// key of partition `part` in the low (even part) or high (odd part) 16 bits
+// of word part / 2, as bound * 32 + part / 2.  For 8-bit texels a bound is at
+// most sqrt(2 * 4 * 255^2 * 16/4 / 65536) * 256 < 1450, so a key is below
+// 1450 * 32 < 2^16 and 0xFFFF (a taken entry) sorts after all of them.
+// Within one half the keys order like the reference's part + bound * 64,
+// i.e. by (bound, part), since all its parts have the same parity.
+struct KeyRow {
+  uint32_t* p;
+  int stride;
+  BCN_HD void put(int part, uint32_t v16) const {
+    reinterpret_cast<uint16_t*>(p + (part >> 1) * stride)[part & 1] = (uint16_t)v16;
+  }
+};
+
 // PCA split bounds of the 64 two-subset partitions (bc7.rs:629-638,
-// 824-843): keys[part * stride] = (int)bound.  For 8-bit texels a bound is at
-// most sqrt(2 * 4 * 255^2 * 16/4 / 65536) * 256 < 1450, so it fits 16 bits
-// and the reference's key part + bound * 64 orders like (bound, part).
+// 824-843) into the key row.
 template <int CH>
-BCN_HD void partition_bounds(const uint32_t px[16], uint16_t* keys, int stride) {
+BCN_HD void partition_bounds(const uint32_t px[16], KeyRow keys) {
   uint32_t cm[4][4];
   channel_major(px, cm);
-  Moments full = moments_cm<CH>(cm, 0xFFFFu);
+  // The block's moments as floats, converted once: every moment is an
+  // integer below 2^24, so float differences of them are exact.
+  float ffs[4], ffq[10];
+  {
+    const Moments full = moments_cm<CH>(cm, 0xFFFFu);
+#pragma unroll
+    for (int i = 0; i < 4; i++) ffs[i] = (float)full.s[i];
+#pragma unroll
+    for (int i = 0; i < 10; i++) ffq[i] = (float)full.q[i];
+  }
 #pragma unroll 1
   for (int part = 0; part < 64; part++) {
     uint32_t mask = kPart.mask01[part] & 0xFFFFu;
     Moments m = moments_cm<CH>(cm, mask);
+    float fs0[4], fq0[10];
+#pragma unroll
+    for (int i = 0; i < 4; i++) fs0[i] = (float)m.s[i];
+#pragma unroll
+    for (int i = 0; i < 10; i++) fq0[i] = (float)m.q[i];
     float bound = 0.0f;
 #pragma unroll
     for (int side = 0; side < 2; side++) {
-      int sm[4], q[10], n;
+      float fs[4], fq[10];
+      int n;
       if (side == 0) {
         n = m.n;
 #pragma unroll
-        for (int i = 0; i < 4; i++) sm[i] = m.s[i];
+        for (int i = 0; i < 4; i++) fs[i] = fs0[i];
 #pragma unroll
-        for (int i = 0; i < 10; i++) q[i] = m.q[i];
+        for (int i = 0; i < 10; i++) fq[i] = fq0[i];
       } else {
-        // full_stats - stats: exact integer differences.
-        n = full.n - m.n;
+        // full_stats - stats: exact differences of integers below 2^24, taken
+        // in float, which saves converting a second set of moments.
+        n = 16 - m.n;
 #pragma unroll
-        for (int i = 0; i < 4; i++) sm[i] = full.s[i] - m.s[i];
+        for (int i = 0; i < 4; i++) fs[i] = ffs[i] - fs0[i];
 #pragma unroll
-        for (int i = 0; i < 10; i++) q[i] = full.q[i] - m.q[i];
+        for (int i = 0; i < 10; i++) fq[i] = ffq[i] - fq0[i];
       }
       float c[10];
       // n in [1, 15] and the same in every lane (the shape is the loop's).
       const float fn = kPartCounts.n[part][side], rn = kPartCounts.rcp[part][side];
-      if ((n & (n - 1)) == 0) covar_from_moments<CH, true>(c, sm, q, fn, rn);
-      else covar_from_moments<CH>(c, sm, q, fn, rn);
+      if ((n & (n - 1)) == 0) covar_from_fmoments<CH, true>(c, fs, fq, fn, rn);
+      else covar_from_fmoments<CH>(c, fs, fq, fn, rn);
       // get_pca_bound (common.rs:208-245).
       const float inv_var = 1.0f / (256.0f * 256.0f);
       const float eps = 0.001f * 0.001f;
@@ -1185,24 +1223,28 @@ BCN_HD void partition_bounds(const uint32_t px[16], uint16_t* keys, int stride) 
       bound = side == 0 ? bb : bound + bb;
     }
     float bound12 = sqrt_cr(bound) * 256.0f;
-    keys[part * stride] = (uint16_t)(int)bound12;  // finite, 0 <= bound12 < 1450
+    // finite, 0 <= (int)bound12 < 1450
+    keys.put(part, (uint32_t)(int)bound12 * 32u + (uint32_t)(part >> 1));
   }
 }
 
-// Next partition in ascending key order (partial_sort_list, common.rs:126-140,
-// consumed one entry at a time); marks it taken with 0xFFFF, which sorts after
-// every real key.
-BCN_HD int select_next(uint16_t* keys, int stride) {
-  // key * 64 + i is unique, so its minimum is the first smallest key.
+// Next partition in ascending (bound, part) order -- the reference's
+// partial_sort_list of part + bound * 64 (common.rs:126-140), consumed one
+// entry at a time; marks it taken with 0xFFFF.  One packed minimum per word
+// gives the smallest even-part key (low half) and odd-part key (high half);
+// keys are unique within a half.  Between the two, lo = bound_a * 32 + w_a
+// (part 2 w_a) and hi = bound_b * 32 + w_b (part 2 w_b + 1) compare like
+// (bound, part): equal bounds and w_a <= w_b mean 2 w_a < 2 w_b + 1, and
+// w_a > w_b means 2 w_b + 1 < 2 w_a.  So lo <= hi picks the even part.
+BCN_HD int select_next(KeyRow keys) {
   uint32_t best = 0xFFFFFFFFu;
 #pragma unroll 8
-  for (int i = 0; i < 64; i++) {
-    const uint32_t key = (uint32_t)keys[i * stride] * 64u + (uint32_t)i;
-    best = key < best ? key : best;
-  }
-  const int best_i = (int)(best & 63u);
-  keys[best_i * stride] = 0xFFFFu;
-  return best_i;
+  for (int w = 0; w < 32; w++) best = pk_min_u16(best, keys.p[w * keys.stride]);
+  const uint32_t lo = best & 0xFFFFu, hi = best >> 16;
+  const bool even = lo <= hi;
+  const int part = (int)(((even ? lo : hi) & 31u) * 2u + (even ? 0u : 1u));
+  keys.put(part, 0xFFFFu);
+  return part;
 }
 
 // fminf / fmaxf into a float in LDS: one non-returning LDS atomic (only the
@@ -1292,18 +1334,18 @@ BCN_HD void bc7_segment_pair3(const uint32_t px[16], uint32_t mask0, float e[2][
 // mode 3 the t3 smallest, from the same ascending list; a partition's two
 // PCA fits (3 channels, its two subset masks) are identical for both modes,
 // so each listed partition is fitted once and quantised for both.
-BCN_HD void enc_mode13(const uint32_t px[16], const Bc7Params& s, uint16_t* keys, int stride,
-                       SubsetRow row, int opaque_err, Bc7Best& best) {
+BCN_HD void enc_mode13(const uint32_t px[16], const Bc7Params& s, KeyRow keys, SubsetRow row,
+                       int opaque_err, Bc7Best& best) {
   const int t1 = (int)s.fast_skip_threshold_mode1, t3 = (int)s.fast_skip_threshold_mode3;
   if (t1 == 0 && t3 == 0) return;
-  partition_bounds<3>(px, keys, stride);
+  partition_bounds<3>(px, keys);
   const int n = t1 > t3 ? t1 : t3;
   PartFit b1, b3;
   b1.err = b3.err = 0x7FFFFFFF;
   int p1 = 0, p3 = 0;
 #pragma unroll 1
   for (int r = 0; r < n; r++) {
-    int part = select_next(keys, stride);
+    int part = select_next(keys);
     float e[3][2][4];
     bc7_segment_pair3(px, part_mask(part, 0), e, row);
     if (r < t1) {
@@ -1336,17 +1378,17 @@ BCN_HD void enc_mode13(const uint32_t px[16], const Bc7Params& s, uint16_t* keys
 
 // bc7_enc_mode7 (bc7.rs:819-851).
 template <int CH>
-BCN_HD void enc_mode7(const uint32_t px[16], const Bc7Params& s, uint16_t* keys, int stride,
-                      SubsetRow row, Bc7Best& best) {
+BCN_HD void enc_mode7(const uint32_t px[16], const Bc7Params& s, KeyRow keys, SubsetRow row,
+                      Bc7Best& best) {
   const int t7 = (int)s.fast_skip_threshold_mode7;
   if (t7 == 0) return;
-  partition_bounds<CH>(px, keys, stride);
+  partition_bounds<CH>(px, keys);
   PartFit bf;
   bf.err = 0x7FFFFFFF;
   int p7 = 0;
 #pragma unroll 1
   for (int r = 0; r < t7; r++) {
-    int part = select_next(keys, stride);
+    int part = select_next(keys);
     PartFit f;
     fit_partition<7>(px, part, f, row);
     if (f.err < bf.err) {
@@ -1603,10 +1645,11 @@ BCN_HD void code_mode6(const Bc7Best& b, uint32_t out[4]) {
 }
 
 // compress_block_bc7_core (bc7.rs:853-867) for one block.  `keys` is a
-// 64-entry scratch row (16-bit) and `sdata` an 18-word one (SubsetRow), both
-// read and written with `stride`.
+// 32-word scratch row (the partition keys of modes 1/3/7 as KeyRow, the kept
+// subset fits of modes 0/2) and `sdata` an 18-word one (SubsetRow), both read
+// and written with `stride`.
 template <int CH>
-BCN_HD void encode_bc7_block(const uint32_t px[16], const Bc7Params& s, uint16_t* keys, int stride,
+BCN_HD void encode_bc7_block(const uint32_t px[16], const Bc7Params& s, uint32_t* keys, int stride,
                              uint32_t* sdata, uint32_t out[4]) {
   const SubsetRow row{sdata, stride};
   Bc7Best best;
@@ -1639,10 +1682,10 @@ BCN_HD void encode_bc7_block(const uint32_t px[16], const Bc7Params& s, uint16_t
 #pragma unroll
   for (int k = 0; k < 16; k++) t[k] = fresh(t[k]);
   if (s.mode_selection[1] != 0) {
-    enc_mode13(t, s, keys, stride, row, opaque_err, best);
+    enc_mode13(t, s, KeyRow{keys, stride}, row, opaque_err, best);
 #pragma unroll
     for (int k = 0; k < 16; k++) t[k] = fresh(t[k]);
-    enc_mode7<CH>(t, s, keys, stride, row, best);
+    enc_mode7<CH>(t, s, KeyRow{keys, stride}, row, best);
   }
 #pragma unroll
   for (int k = 0; k < 16; k++) t[k] = fresh(t[k]);

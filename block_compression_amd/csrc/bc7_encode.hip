@@ -23,7 +23,7 @@ __attribute__((amdgpu_waves_per_eu(BCN_BC7_WAVES, BCN_BC7_WAVES))) void bc7_enco
                                                               uint32_t bw, uint32_t nblocks,
                                                               uint32_t* __restrict__ dst,
                                                               Bc7Params s) {
-  __shared__ uint16_t keys[64 * kBc7Wave];
+  __shared__ uint32_t keys[32 * kBc7Wave];
   __shared__ uint32_t sdata[18 * kBc7Wave];
   uint32_t idx = blockIdx.x * kBc7Wave + threadIdx.x;
   if (idx >= nblocks) return;

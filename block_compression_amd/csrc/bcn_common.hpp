@@ -111,6 +111,19 @@ BCN_HD uint32_t pk_wdelta16(uint32_t w, uint32_t d) {
 #endif
 }
 
+// Lane-wise minimum of unsigned 16-bit lanes (v_pk_min_u16).
+BCN_HD uint32_t pk_min_u16(uint32_t a, uint32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(u16x2, a),
+                                                                 __builtin_bit_cast(u16x2, b)));
+#else
+  const uint32_t lo = (a & 0xFFFFu) < (b & 0xFFFFu) ? (a & 0xFFFFu) : (b & 0xFFFFu);
+  const uint32_t hi = (a >> 16) < (b >> 16) ? (a >> 16) : (b >> 16);
+  return lo | (hi << 16);
+#endif
+}
+
 // Lane-wise logical shifts of unsigned 16-bit lanes by per-lane amounts
 // (v_pk_lshlrev_b16 / v_pk_lshrrev_b16); amounts below 16.
 BCN_HD uint32_t pk_shl16(uint32_t v, uint32_t sh) {

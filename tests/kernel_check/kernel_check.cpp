@@ -21,7 +21,7 @@ extern "C" int kcheck_bc7_encode(const void* settings, const uint8_t* rgba, uint
         px[y * 4 + x] = (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) |
                         ((uint32_t)p[3] << 24);
       }
-    uint16_t keys[64];
+    uint32_t keys[32];
     uint32_t sdata[18];
     uint32_t o[4];
     if (s.channels == 4) bcn::encode_bc7_block<4>(px, s, keys, 1, sdata, o);
