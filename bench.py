@@ -564,7 +564,8 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=10,
+                    help="untimed steps before the timed ones (the engine clock ramps up over the first ~6)")
     ap.add_argument("--parity-blocks", type=int, default=-1,
                     help="blocks checked against the CPU oracle: -1 = every block (default), "
                          "n > 0 = a seeded sample of n, 0 = none")

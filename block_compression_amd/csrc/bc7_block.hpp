@@ -660,6 +660,82 @@ BCN_HD int block_quant(const uint32_t px[16], const int d[3][2][4], uint32_t pat
   return total;
 }
 
+// block_quant of a mode 4/5 candidate's colour part for a 3-channel block
+// when the rotated channel's dequantised endpoints are both 255.  The rotated
+// block holds 255 in that channel for every texel (bc7.rs:690-695), so the
+// channel adds (255 - 255) * 0 to every projection and to div, and its
+// interpolated value is 255 for every index: its error is 0.  The search is
+// then block_quant on the two other channels alone, in one pair of 16-bit
+// lanes: a2 / b2 hold their endpoints 0 / 1 (low lane, high lane), and `sel`
+// is the v_perm_b32 selector that takes their bytes from a texel word.  Same
+// projections, thresholds, candidate errors and ties as block_quant, so the
+// same indices and SSE.
+template <int BITS>
+BCN_HD int block_quant_pair(const uint32_t px[16], uint32_t a2, uint32_t b2, uint32_t sel,
+                            uint32_t qblock[2]) {
+  launder_texels(px);
+  constexpr int L = 1 << BITS;
+  static_assert(L == 4 || L == 8, "2- and 3-bit indices (modes 4 and 5)");
+  const uint32_t de = pk_sub16(b2, a2);
+  const int div = dot2_16(de, de, 0);
+  int t2 = 0, t3 = 0, c1 = 0;
+  float rq = 0.0f;
+  if (L == 4) quant4_thresholds(div, t2, t3);
+  else quant8_consts(div, c1, rq);
+  // proj = t.de - a.de and the rounding term (32 + 64 a) - 64 t, as the
+  // one-subset block_quant forms them.
+  const int nk = -dot2_16(a2, de, 0);
+  const uint32_t base = pk_mad16(a2, 0x00400040u, 0x00200020u);
+  const uint32_t m64 = fresh(0xFFC0FFC0u);
+  int total = 0;
+  qblock[0] = 0;
+  qblock[1] = 0;
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    const uint32_t tp = perm_b32(0u, px[k], sel);
+    const int proj = dot2_16(tp, de, nk);
+    const uint32_t c = pk_mad16(tp, m64, base);
+    const int q1 = L == 4 ? 1 + (int)(proj >= t2) + (int)(proj >= t3) : quant8_index(proj, c1, rq);
+    const uint32_t wp = weight_pair<BITS>(q1);
+    const uint32_t t0 = pk_wmad16<0>(wp, de, c), t1 = pk_wmad16<1>(wp, de, c);
+    const int err0 = dot2_16(t0, t0, total);
+    const int err1 = dot2_16(t1, t1, total);
+    const bool lo = err0 < err1;
+    total = lo ? err0 : err1;
+    qblock[k >> 3] |= (uint32_t)(q1 - (int)lo) << (4 * (k & 7));
+  }
+  return total;
+}
+
+// The colour-part block_quant of a mode 4/5 candidate: block_quant_pair when
+// every lane of the wave has the rotated channel at 255/255 (3-channel
+// rotation, ROT 1), else the full 3-channel search of the rotated block.
+template <int BITS, int ROT>
+BCN_HD int block_quant45(const uint32_t px[16], const int d[3][2][4], uint32_t rot, uint32_t sh,
+                         uint32_t qblock[2]) {
+  if (ROT == 1) {
+    // The endpoints as plain registers first (fresh): a select between
+    // elements of d would otherwise become a run-time index into d, which
+    // moves the array to private memory.
+    uint32_t a[3], b[3];
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+      a[p] = fresh((uint32_t)d[0][0][p]);
+      b[p] = fresh((uint32_t)d[0][1][p]);
+    }
+    const uint32_t ar = rot == 0 ? a[0] : (rot == 1 ? a[1] : a[2]);
+    const uint32_t br = rot == 0 ? b[0] : (rot == 1 ? b[1] : b[2]);
+    if (wave_all(ar == 255u && br == 255u)) {
+      // The other two channels, in channel order: (G, B), (R, B) or (R, G).
+      const uint32_t a2 = rot == 0 ? (a[1] | (a[2] << 16)) : (rot == 1 ? (a[0] | (a[2] << 16)) : (a[0] | (a[1] << 16)));
+      const uint32_t b2 = rot == 0 ? (b[1] | (b[2] << 16)) : (rot == 1 ? (b[0] | (b[2] << 16)) : (b[0] | (b[1] << 16)));
+      const uint32_t sel = rot == 0 ? 0x0C020C01u : (rot == 1 ? 0x0C020C00u : 0x0C010C00u);
+      return block_quant_pair<BITS>(px, a2, b2, sel, qblock);
+    }
+  }
+  return block_quant<3, BITS, 1, ROT>(px, d, 0u, qblock, sh);
+}
+
 // opt_endpoints (common.rs:142-205): least-squares endpoints of the masked
 // texels for fixed indices.  Sums are integers (<= 16*15*255 = 61,200);
 // cxx, cyy, cxy and the determinant are exact small integers (det <=
@@ -1415,12 +1491,12 @@ BCN_HD void mode45_candidate(const uint32_t px[16], uint32_t sh, const float se0
   int q[2][4], d[3][2][4];
   quant_pair<MODE, 3>(e0, e1, q[0], q[1], d[0][0], d[0][1]);
   uint32_t qblock[2];
-  int err = block_quant<3, BITS, 1, ROT>(px, d, 0u, qblock, sh);
+  int err = block_quant45<BITS, ROT>(px, d, rotation, sh, qblock);
 #pragma unroll 1
   for (uint32_t it = 0; it < s.refine_iterations[MODE]; it++) {
     opt_endpoints<3, BITS, ROT>(px, qblock, 0xFFFFu, e0, e1, sh);
     quant_pair<MODE, 3>(e0, e1, q[0], q[1], d[0][0], d[0][1]);
-    err = block_quant<3, BITS, 1, ROT>(px, d, 0u, qblock, sh);
+    err = block_quant45<BITS, ROT>(px, d, rotation, sh, qblock);
   }
   ChannelFit a = opt_channel<ABITS, AEPBITS>(px, sh, s.refine_iterations_channel);
   err += a.err;

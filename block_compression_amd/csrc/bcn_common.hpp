@@ -111,6 +111,24 @@ BCN_HD uint32_t pk_wdelta16(uint32_t w, uint32_t d) {
 #endif
 }
 
+// Whether any lane of the wave has `p` (the lane's own value on the host).
+BCN_HD bool wave_any(bool p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __ballot(p) != 0;
+#else
+  return p;
+#endif
+}
+
+// Whether every lane of the wave has `p` (the lane's own value on the host).
+BCN_HD bool wave_all(bool p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __ballot(!p) == 0;
+#else
+  return p;
+#endif
+}
+
 // Lane-wise minimum of unsigned 16-bit lanes (v_pk_min_u16).
 BCN_HD uint32_t pk_min_u16(uint32_t a, uint32_t b) {
 #if defined(__HIP_DEVICE_COMPILE__)

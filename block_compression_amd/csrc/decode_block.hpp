@@ -445,24 +445,6 @@ struct Bc6hScratch {
   BCN_HD uint32_t& at(int w) const { return row[w * stride]; }
 };
 
-// Whether any lane of the wave has `p` (the lane's own value on the host).
-BCN_HD bool wave_any(bool p) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  return __ballot(p) != 0;
-#else
-  return p;
-#endif
-}
-
-// Whether every lane of the wave has `p` (the lane's own value on the host).
-BCN_HD bool wave_all(bool p) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  return __ballot(!p) == 0;
-#else
-  return p;
-#endif
-}
-
 // Mode index of a BC6H block (0..13; 14 for the reserved prefixes) from its
 // 2- or 5-bit prefix (block.rs:270-290).
 BCN_HD int bc6h_mode(const uint32_t in[4]) {
