@@ -1264,7 +1264,7 @@ BCN_HD void partition_bounds(const uint32_t px[16], KeyRow keys) {
       }
       float c[10];
       // n in [1, 15] and the same in every lane (the shape is the loop's).
-      const float fn = kPartCounts.n[part][side], rn = kPartCounts.rcp[part][side];
+      const float fn = in_vgpr(kPartCounts.n[part][side]), rn = in_vgpr(kPartCounts.rcp[part][side]);
       if ((n & (n - 1)) == 0) covar_from_fmoments<CH, true>(c, fs, fq, fn, rn);
       else covar_from_fmoments<CH>(c, fs, fq, fn, rn);
       // get_pca_bound (common.rs:208-245).

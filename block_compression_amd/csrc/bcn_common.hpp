@@ -228,23 +228,42 @@ BCN_HD float div_rcp(float a, float b, float y) {
   return __builtin_fmaf(r, y, q);
 }
 
-// sqrtf(x), correctly rounded, for every x.  The hardware square root
-// followed by the test of the candidates one ulp below and above with an
-// exact FMA residual (the correction the compiler's sequence applies, without
-// its rescaling of tiny inputs and its class fix-ups) equals sqrtf for x == 0
-// and every x >= 2^-104, as an exhaustive sweep of all non-negative floats on
-// the GPU shows (tests/test_gpu_fastmath.py); other inputs (tiny, negative,
-// infinite, NaN) take the compiler's sequence, a branch no wave takes in
-// practice.
+#if defined(__HIP_DEVICE_COMPILE__)
+// The square root of x >= 2^-100 (finite), correctly rounded, from the
+// hardware reciprocal square root y ~ 1/sqrt(x) by Markstein's method (P.
+// Markstein, "IA-64 and Elementary Functions", 2000, ch. 8): g = x y and
+// h = y / 2 approximate sqrt(x) and 1/(2 sqrt(x)); one coupled correction
+// r = 1/2 - g h refines both, and the final FMA from the exact residual
+// x - g^2 rounds once.  Equal to sqrtf on every float from 2^-102 up and at 0
+// (exhaustive GPU sweep, tests/test_gpu_fastmath.py); the callers pass 0 as
+// the smallest normal argument, which gives g = 0 and a result of 0.
+// Seven full-rate f32 operations and one transcendental, against the
+// compiler's correction of the hardware square root by two candidate
+// residuals (compares and selects).
+__device__ __forceinline__ float sqrt_markstein(float x) {
+  const float y = __builtin_amdgcn_rsqf(x);
+  const float g = x * y, h = 0.5f * y;
+  const float r = __builtin_fmaf(-g, h, 0.5f);
+  const float g1 = __builtin_fmaf(g, r, g), h1 = __builtin_fmaf(h, r, h);
+  const float d = __builtin_fmaf(-g1, g1, x);
+  return __builtin_fmaf(d, h1, g1);
+}
+#endif
+
+// sqrtf(x), correctly rounded, for every x: sqrt_markstein on 0 and on
+// [2^-100, 2^128) (0 as max(x, 2^-100), whose reciprocal square root is
+// finite); other inputs (tiny, negative, infinite, NaN) take the compiler's
+// sequence, a branch no wave takes in practice.
 BCN_HD float sqrt_cr(float x) {
 #if defined(__HIP_DEVICE_COMPILE__)
   if (!(x == 0.0f || (x >= 0x1p-100f && x <= 0x1.fffffep127f))) return sqrtf(x);
-  float s = __builtin_amdgcn_sqrtf(x);
-  const uint32_t b = __float_as_uint(s);
-  const float sd = __uint_as_float(b - 1u), su = __uint_as_float(b + 1u);
-  const float rd = __builtin_fmaf(-sd, s, x), ru = __builtin_fmaf(-su, s, x);
-  s = rd <= 0.0f ? sd : s;
-  return ru > 0.0f ? su : s;
+  const float xs = fmaxf(x, 0x1p-100f);
+  const float y = __builtin_amdgcn_rsqf(xs);
+  const float g = x * y, h = 0.5f * y;
+  const float r = __builtin_fmaf(-g, h, 0.5f);
+  const float g1 = __builtin_fmaf(g, r, g), h1 = __builtin_fmaf(h, r, h);
+  const float d = __builtin_fmaf(-g1, g1, x);
+  return __builtin_fmaf(d, h1, g1);
 #else
   return sqrtf(x);
 #endif
@@ -281,20 +300,14 @@ BCN_HD float rcp_cr(float x) {
 }
 
 // 1.0f / sqrtf(x), both correctly rounded, for every x.  For x in
-// [2^-100, 2^128) the square root is the hardware one with the +-1 ulp FMA
-// correction of sqrt_cr (exact from 2^-104 up, exhaustive GPU sweep) and lies
-// in [2^-50, 2^64], where the hardware reciprocal with one Newton step is the
-// correctly rounded 1/s (same sweep); other inputs take the compiler's
-// sequences, behind one branch.
+// [2^-100, 2^128) the square root is sqrt_markstein and lies in
+// [2^-50, 2^64], where the hardware reciprocal with one Newton step is the
+// correctly rounded 1/s (exhaustive GPU sweep, tests/test_gpu_fastmath.py);
+// other inputs take the compiler's sequences, behind one branch.
 BCN_HD float rsqrt_cr(float x) {
 #if defined(__HIP_DEVICE_COMPILE__)
   if (!(x >= 0x1p-100f && x <= 0x1.fffffep127f)) return 1.0f / sqrtf(x);
-  float s = __builtin_amdgcn_sqrtf(x);
-  const uint32_t b = __float_as_uint(s);
-  const float sd = __uint_as_float(b - 1u), su = __uint_as_float(b + 1u);
-  const float rd = __builtin_fmaf(-sd, s, x), ru = __builtin_fmaf(-su, s, x);
-  s = rd <= 0.0f ? sd : s;
-  s = ru > 0.0f ? su : s;
+  const float s = sqrt_markstein(x);
   const float y = __builtin_amdgcn_rcpf(s);
   return __builtin_fmaf(__builtin_fmaf(-s, y, 1.0f), y, y);
 #else
@@ -314,6 +327,24 @@ BCN_HD uint32_t fresh(uint32_t x) {
   asm volatile("" : "+v"(x));
 #else
   asm volatile("" : "+r"(x));
+#endif
+  return x;
+}
+
+// A wave-uniform value copied into a vector register.  On gfx950 a VALU
+// instruction of the fast kinds (f32 add/mul/fma, integer add, logic) issues
+// every ~2.4 cycles per SIMD with vector or constant operands but every ~4.2
+// when one operand is a scalar register; a value used by many such
+// instructions is cheaper in a vector register (one copy).
+BCN_HD float in_vgpr(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm volatile("" : "+v"(x));
+#endif
+  return x;
+}
+BCN_HD uint32_t in_vgpr(uint32_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm volatile("" : "+v"(x));
 #endif
   return x;
 }

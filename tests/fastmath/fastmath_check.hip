@@ -16,12 +16,11 @@
 #include "../../block_compression_amd/csrc/decode_block.hpp"
 
 __device__ float fast_sqrt_unguarded(float x) {
-  float s = __builtin_amdgcn_sqrtf(x);
-  const uint32_t b = __float_as_uint(s);
-  const float sd = __uint_as_float(b - 1u), su = __uint_as_float(b + 1u);
-  const float rd = __builtin_fmaf(-sd, s, x), ru = __builtin_fmaf(-su, s, x);
-  s = rd <= 0.0f ? sd : s;
-  return ru > 0.0f ? su : s;
+#if defined(__HIP_DEVICE_COMPILE__)
+  return bcn::sqrt_markstein(x);
+#else
+  return x;
+#endif
 }
 
 __device__ float fast_rcp_unguarded(float x) {
