@@ -819,6 +819,37 @@ BCN_HD int chan(const uint32_t px[16], int k, uint32_t sh) {
   return (int)((px[k] >> sh) & 0xFFu);  // px laundered by the caller
 }
 
+// The first index candidate of channel_opt_quant's 2-bit search,
+// q1 = clamp((int)(fl(fl(u / den) * 4) + 0.5), 1, 3) for u = texel - e0 and
+// den = fl(d + 0.001f), d = e1 - e0 (u, d integers in [-255, 255]), is
+// 1 + [y >= 2] + [y >= 3] with y the rounded sum.  With r = u / den the two
+// roundings move y from 4 r + 0.5 by less than 2^-21 (4 |r| + 1), so
+// y >= m exactly when u / den >= t = (2m - 1) / 8 up to a relative 2^-20;
+// and den t = d (2m - 1) / 8 + (0.001f + rho) t, rho the rounding of den
+// (|rho| <= 2^-24 * 255.001), i.e. a multiple of 1/8 plus an offset in
+// [0.00036, 0.00064], which keeps den t at least 0.0003 from every integer
+// while the rounding slack is below 255 * 2^-20 * 0.625 < 0.0002.  So for
+// d >= 0 (den > 0) y >= m exactly when u > T_m = floor(d (2m - 1) / 8), and
+// for d < 0 exactly when u <= T_m.  Both are sign tests on a 16-bit sum in
+// the high half of a word: with the texel in both halves (v2 = v | v << 16),
+// (v2 ^ s) + k_m has bit 31 set exactly then, where s = 0xFFFF0000 and
+// k_m = (T_m + e0 + 1) << 16 for d >= 0 (the high half is then
+// 65535 - v + T_m + e0 + 1, below 2^16 exactly when v > T_m + e0), s = 0 and
+// k_m = -(T_m + e0 + 1) << 16 for d < 0; the low half (v) never carries, and
+// every sum stays within 16-bit range.  tests/kernel_check checks every
+// (e0, e1, texel) against the float form.
+BCN_HD void chan_q4_consts(int e0, int e1, uint32_t& s, uint32_t& k2, uint32_t& k3) {
+  const int d = e1 - e0;
+  const int t2 = ((3 * d) >> 3) + e0 + 1, t3 = ((5 * d) >> 3) + e0 + 1;
+  s = d >= 0 ? 0xFFFF0000u : 0u;
+  k2 = (uint32_t)(d >= 0 ? t2 : -t2) << 16;
+  k3 = (uint32_t)(d >= 0 ? t3 : -t3) << 16;
+}
+BCN_HD int chan_q4_index(uint32_t v2, uint32_t s, uint32_t k2, uint32_t k3) {
+  const uint32_t x = v2 ^ s;
+  return 1 + (int)((x + k2) >> 31) + (int)((x + k3) >> 31);
+}
+
 // channel_opt_quant (bc7.rs:278-320).
 template <int BITS>
 BCN_HD int channel_quant(const uint32_t px[16], uint32_t sh, int e0, int e1, uint32_t qblock[2]) {
@@ -827,6 +858,9 @@ BCN_HD int channel_quant(const uint32_t px[16], uint32_t sh, int e0, int e1, uin
   // |den| in [0.001, 255.001]
   float den = (float)(e1 - e0) + 0.001f, rden = rcp_normal(den);
   const float fe0 = (float)e0;
+  // 2-bit searches: the first candidate from two sign tests (chan_q4_consts).
+  uint32_t q4s = 0, q4k2 = 0, q4k3 = 0;
+  if (BITS == 2) chan_q4_consts(e0, e1, q4s, q4k2, q4k3);
   qblock[0] = 0;
   qblock[1] = 0;
   // Both candidates' interpolated-minus-texel values in the two 16-bit lanes:
@@ -840,11 +874,16 @@ BCN_HD int channel_quant(const uint32_t px[16], uint32_t sh, int e0, int e1, uin
 #pragma unroll
   for (int k = 0; k < 16; k++) {
     const uint32_t cv2 = perm_b32(0u, px[k], sel);
-    // (float)(texel - e0) as a difference of exact small integers.
-    float proj = div_rcp(chf(cv2, 0) - fe0, den, rden);
-    // proj * L is exact (|proj| <= 255000, a power-of-two factor), so the
-    // fused form rounds once, as proj * L + 0.5 does.
-    int q1 = qclamp(__builtin_fmaf(proj, (float)L, 0.5f), 1.0f, (float)(L - 1));
+    int q1;
+    if (BITS == 2) {
+      q1 = chan_q4_index(cv2, q4s, q4k2, q4k3);
+    } else {
+      // (float)(texel - e0) as a difference of exact small integers.
+      float proj = div_rcp(chf(cv2, 0) - fe0, den, rden);
+      // proj * L is exact (|proj| <= 255000, a power-of-two factor), so the
+      // fused form rounds once, as proj * L + 0.5 does.
+      q1 = qclamp(__builtin_fmaf(proj, (float)L, 0.5f), 1.0f, (float)(L - 1));
+    }
     const uint32_t t = pk_mad_shr6(weight_pair<BITS>(q1), de2, pk_mad16(cv2, m64, base2));
     // |t| <= 255, so the squares fit unsigned 16-bit lanes.  Each error on
     // top of the running total (sums <= 16 * 255^2); the smaller is the new

@@ -251,13 +251,16 @@ __device__ __forceinline__ float sqrt_markstein(float x) {
 #endif
 
 // sqrtf(x), correctly rounded, for every x: sqrt_markstein on 0 and on
-// [2^-100, 2^128) (0 as max(x, 2^-100), whose reciprocal square root is
-// finite); other inputs (tiny, negative, infinite, NaN) take the compiler's
+// [2^-100, 2^128) (with the reciprocal square root of 2^-100 for 0, which
+// keeps it finite); other inputs (tiny, negative, infinite, NaN) take the compiler's
 // sequence, a branch no wave takes in practice.
 BCN_HD float sqrt_cr(float x) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  if (!(x == 0.0f || (x >= 0x1p-100f && x <= 0x1.fffffep127f))) return sqrtf(x);
-  const float xs = fmaxf(x, 0x1p-100f);
+  // x in [2^-100, 2^128) as one unsigned range test on the bit pattern
+  // (negative x, +-inf and NaN fall outside it), or +0.
+  const uint32_t u = __float_as_uint(x);
+  if (!(u - 0x0D800000u <= 0x7F7FFFFFu - 0x0D800000u || u == 0u)) return sqrtf(x);
+  const float xs = __uint_as_float(u > 0x0D800000u ? u : 0x0D800000u);
   const float y = __builtin_amdgcn_rsqf(xs);
   const float g = x * y, h = 0.5f * y;
   const float r = __builtin_fmaf(-g, h, 0.5f);
@@ -306,7 +309,8 @@ BCN_HD float rcp_cr(float x) {
 // other inputs take the compiler's sequences, behind one branch.
 BCN_HD float rsqrt_cr(float x) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  if (!(x >= 0x1p-100f && x <= 0x1.fffffep127f)) return 1.0f / sqrtf(x);
+  // x in [2^-100, 2^128): one unsigned range test on the bit pattern.
+  if (!(__float_as_uint(x) - 0x0D800000u <= 0x7F7FFFFFu - 0x0D800000u)) return 1.0f / sqrtf(x);
   const float s = sqrt_markstein(x);
   const float y = __builtin_amdgcn_rcpf(s);
   return __builtin_fmaf(__builtin_fmaf(-s, y, 1.0f), y, y);

@@ -323,6 +323,30 @@ extern "C" long kcheck_quant8_mismatches() {
   return bad;
 }
 
+// chan_q4_consts / chan_q4_index (bc7_block.hpp) against channel_opt_quant's
+// float form of the 2-bit index candidate (bc7.rs:292-295),
+// clamp((int)(fl(fl(fl(texel - e0) / fl(fl(e1 - e0) + 0.001f)) * 4) + 0.5),
+// 1, 3), for every endpoint pair e0, e1 and texel in 0..255.  Returns the
+// number of mismatches.
+extern "C" long kcheck_chan_q4_mismatches() {
+  long bad = 0;
+  for (int e0 = 0; e0 < 256; e0++)
+    for (int e1 = 0; e1 < 256; e1++) {
+      uint32_t s, k2, k3;
+      bcn::chan_q4_consts(e0, e1, s, k2, k3);
+      volatile float den = (float)(e1 - e0) + 0.001f;
+      for (int v = 0; v < 256; v++) {
+        volatile float proj = ((float)v - (float)e0) / den;
+        volatile float y = (float)proj * 4.0f;
+        y = (float)y + 0.5f;
+        int q = (int)(float)y;
+        q = q < 1 ? 1 : (q > 3 ? 3 : q);
+        if (bcn::chan_q4_index((uint32_t)v * 0x10001u, s, k2, k3) != q) bad++;
+      }
+    }
+  return bad;
+}
+
 // block_pos (decode_block.hpp) for n (bw, idx) pairs: rows[i], cols[i].
 extern "C" void kcheck_block_pos(const uint32_t* bw, const uint32_t* idx, size_t n, uint32_t* rows,
                                  uint32_t* cols) {

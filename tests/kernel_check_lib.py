@@ -48,6 +48,12 @@ def quant4_mismatches() -> int:
     return f()
 
 
+def chan_q4_mismatches() -> int:
+    f = lib().kcheck_chan_q4_mismatches
+    f.restype = ctypes.c_long
+    return f()
+
+
 def quant8_mismatches() -> int:
     f = lib().kcheck_quant8_mismatches
     f.restype = ctypes.c_long

@@ -47,6 +47,14 @@ def test_bc7_2bit_index_thresholds_match_float_form():
     assert kc.quant4_mismatches() == 0
 
 
+def test_bc7_channel_2bit_index_sign_tests_match_float_form():
+    # The scalar channel's 2-bit searches (modes 4/5) take their first
+    # candidate from two sign tests on the texel (chan_q4_consts in
+    # bc7_block.hpp); checked against the reference's float form on every
+    # endpoint pair and texel value.
+    assert kc.chan_q4_mismatches() == 0
+
+
 def test_bc7_3bit_index_reciprocal_matches_float_form():
     # The 3-bit index searches take their first candidate from one product
     # with a reciprocal (quant8_index in bc7_block.hpp); both it and the float
