@@ -850,6 +850,24 @@ BCN_HD int chan_q4_index(uint32_t v2, uint32_t s, uint32_t k2, uint32_t k3) {
   return 1 + (int)((x + k2) >> 31) + (int)((x + k3) >> 31);
 }
 
+// The first index candidate of channel_opt_quant's 3-bit search,
+// clamp((int)(fl(fl(u / den) * 8) + 0.5), 1, 7) with u = texel - e0 and
+// den = fl(e1 - e0 + 0.001f), from one multiply-add and one product:
+// clamp(fl(fl(8 texel + c) * r)) with c = fl(den / 2 - 8 e0) and r the
+// correctly rounded 1 / den.  The exact value (8 u + den / 2) / den stays at
+// least 0.0015 / |den| from every integer (8 u is a multiple of 8 and
+// den (m - 1/2) a multiple of 1/2 plus (m - 1/2)(0.001f + rho)), while the
+// three roundings move it by less than 2^-22 * 2200 / |den|.
+// tests/kernel_check checks every (e0, e1, texel) against the float form.
+BCN_HD void chan_q8_consts(int e0, int e1, float& c, float& r) {
+  const float den = (float)(e1 - e0) + 0.001f;  // |den| in [0.001, 255.001]
+  c = 0.5f * den - 8.0f * (float)e0;
+  r = rcp_normal(den);
+}
+BCN_HD int chan_q8_index(float v, float c, float r) {
+  return qclamp(__builtin_fmaf(v, 8.0f, c) * r, 1.0f, 7.0f);
+}
+
 // channel_opt_quant (bc7.rs:278-320).
 template <int BITS>
 BCN_HD int channel_quant(const uint32_t px[16], uint32_t sh, int e0, int e1, uint32_t qblock[2]) {
@@ -861,6 +879,9 @@ BCN_HD int channel_quant(const uint32_t px[16], uint32_t sh, int e0, int e1, uin
   // 2-bit searches: the first candidate from two sign tests (chan_q4_consts).
   uint32_t q4s = 0, q4k2 = 0, q4k3 = 0;
   if (BITS == 2) chan_q4_consts(e0, e1, q4s, q4k2, q4k3);
+  // 3-bit searches: one multiply-add and one product (chan_q8_consts).
+  float q8c = 0.0f, q8r = 0.0f;
+  if (BITS == 3) chan_q8_consts(e0, e1, q8c, q8r);
   qblock[0] = 0;
   qblock[1] = 0;
   // Both candidates' interpolated-minus-texel values in the two 16-bit lanes:
@@ -877,6 +898,8 @@ BCN_HD int channel_quant(const uint32_t px[16], uint32_t sh, int e0, int e1, uin
     int q1;
     if (BITS == 2) {
       q1 = chan_q4_index(cv2, q4s, q4k2, q4k3);
+    } else if (BITS == 3) {
+      q1 = chan_q8_index(chf(cv2, 0), q8c, q8r);
     } else {
       // (float)(texel - e0) as a difference of exact small integers.
       float proj = div_rcp(chf(cv2, 0) - fe0, den, rden);

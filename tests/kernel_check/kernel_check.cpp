@@ -347,6 +347,29 @@ extern "C" long kcheck_chan_q4_mismatches() {
   return bad;
 }
 
+// chan_q8_consts / chan_q8_index (bc7_block.hpp) against channel_opt_quant's
+// float form of the 3-bit index candidate (bc7.rs:292-295) for every
+// endpoint pair e0, e1 and texel in 0..255.  Returns the number of
+// mismatches.
+extern "C" long kcheck_chan_q8_mismatches() {
+  long bad = 0;
+  for (int e0 = 0; e0 < 256; e0++)
+    for (int e1 = 0; e1 < 256; e1++) {
+      float c, r;
+      bcn::chan_q8_consts(e0, e1, c, r);
+      volatile float den = (float)(e1 - e0) + 0.001f;
+      for (int v = 0; v < 256; v++) {
+        volatile float proj = ((float)v - (float)e0) / den;
+        volatile float y = (float)proj * 8.0f;
+        y = (float)y + 0.5f;
+        int q = (int)(float)y;
+        q = q < 1 ? 1 : (q > 7 ? 7 : q);
+        if (bcn::chan_q8_index((float)v, c, r) != q) bad++;
+      }
+    }
+  return bad;
+}
+
 // block_pos (decode_block.hpp) for n (bw, idx) pairs: rows[i], cols[i].
 extern "C" void kcheck_block_pos(const uint32_t* bw, const uint32_t* idx, size_t n, uint32_t* rows,
                                  uint32_t* cols) {

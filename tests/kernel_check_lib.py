@@ -54,6 +54,12 @@ def chan_q4_mismatches() -> int:
     return f()
 
 
+def chan_q8_mismatches() -> int:
+    f = lib().kcheck_chan_q8_mismatches
+    f.restype = ctypes.c_long
+    return f()
+
+
 def quant8_mismatches() -> int:
     f = lib().kcheck_quant8_mismatches
     f.restype = ctypes.c_long

@@ -55,6 +55,14 @@ def test_bc7_channel_2bit_index_sign_tests_match_float_form():
     assert kc.chan_q4_mismatches() == 0
 
 
+def test_bc7_channel_3bit_index_product_matches_float_form():
+    # The scalar channel's 3-bit search (mode 4, index selector 0) takes its
+    # first candidate from one multiply-add and one product (chan_q8_consts);
+    # checked against the reference's float form on every endpoint pair and
+    # texel value.
+    assert kc.chan_q8_mismatches() == 0
+
+
 def test_bc7_3bit_index_reciprocal_matches_float_form():
     # The 3-bit index searches take their first candidate from one product
     # with a reciprocal (quant8_index in bc7_block.hpp); both it and the float
