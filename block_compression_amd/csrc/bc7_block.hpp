@@ -159,18 +159,23 @@ BCN_HD Moments moments(const uint32_t px[16], uint32_t mask, uint32_t sh = 0) {
 }
 
 // moments() from channel-major bytes: the masked channel bytes are one AND
-// with the group's byte mask, then one v_dot4_u32_u8 per sum.
-template <int CH>
+// with the group's byte mask, then one v_dot4_u32_u8 per sum.  UNIFORM: the
+// mask is the same in every lane (the shape loops of modes 0/2 and the
+// partition bounds), so a group of four texels outside the mask is skipped
+// and one inside it needs no AND, by scalar branches.
+template <int CH, bool UNIFORM = false>
 BCN_HD Moments moments_cm(const uint32_t cm[4][4], uint32_t mask) {
   Moments m{};
   m.n = __builtin_popcount(mask & 0xFFFFu);
   uint32_t s[4] = {0, 0, 0, 0}, q[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
   for (int g = 0; g < 4; g++) {
-    const uint32_t mb = ((((mask >> (4 * g)) & 0xFu) * 0x00204081u) & 0x01010101u) * 0xFFu;
+    const uint32_t nib = (mask >> (4 * g)) & 0xFu;
+    if (UNIFORM && nib == 0u) continue;
+    const uint32_t mb = ((nib * 0x00204081u) & 0x01010101u) * 0xFFu;
     uint32_t x[4];
 #pragma unroll
-    for (int p = 0; p < CH; p++) x[p] = cm[g][p] & mb;
+    for (int p = 0; p < CH; p++) x[p] = (UNIFORM && nib == 0xFu) ? cm[g][p] : cm[g][p] & mb;
     s[0] = udot4(x[0], 0x01010101u, s[0]);
     s[1] = udot4(x[1], 0x01010101u, s[1]);
     s[2] = udot4(x[2], 0x01010101u, s[2]);
@@ -1225,7 +1230,7 @@ BCN_HD void fit_partition_planned(const uint32_t px[16], int part, uint32_t* row
       // bc7_segment<3>, its moments from the channel-major copy.
       const uint32_t mask = part_mask(part_id, j);
       float e0[4], e1[4], axis[4], dc[4];
-      pca_axis_moments<3>(moments_cm<3>(cm, mask), axis, dc);
+      pca_axis_moments<3>(moments_cm<3, true>(cm, mask), axis, dc);
       pca_extent<3>(px, mask, axis, dc, e0, e1);
 #pragma unroll
       for (int p = 0; p < 3; p++) {
@@ -1298,7 +1303,7 @@ BCN_HD void partition_bounds(const uint32_t px[16], KeyRow keys) {
 #pragma unroll 1
   for (int part = 0; part < 64; part++) {
     uint32_t mask = kPart.mask01[part] & 0xFFFFu;
-    Moments m = moments_cm<CH>(cm, mask);
+    Moments m = moments_cm<CH, true>(cm, mask);
     float fs0[4], fq0[10];
 #pragma unroll
     for (int i = 0; i < 4; i++) fs0[i] = (float)m.s[i];
