@@ -202,9 +202,15 @@ BCN_HD uint32_t udot4(uint32_t a, uint32_t b, uint32_t c) {
 }
 
 // a.lo * b.lo + a.hi * b.hi + c with signed lanes and 32-bit products.
+// Every caller's sum stays well inside int32, so the clamp bit (saturation
+// of the result) never acts; it makes the compiler select the three-operand
+// v_dot2_i32_i16, whose accumulator may be any register or an inline
+// constant, instead of the two-operand v_dot2c_i32_i16, which needs a copy
+// of the accumulator into the destination whenever the accumulator is a
+// constant or is still live (one extra move per dot-product chain).
 BCN_HD int dot2_16(uint32_t a, uint32_t b, int c) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  return __builtin_amdgcn_sdot2(as_s16x2(a), as_s16x2(b), c, false);
+  return __builtin_amdgcn_sdot2(as_s16x2(a), as_s16x2(b), c, true);
 #else
   return lane16(a, 0) * lane16(b, 0) + lane16(a, 1) * lane16(b, 1) + c;
 #endif
