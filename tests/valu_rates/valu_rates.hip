@@ -85,7 +85,6 @@ DEFK(v_mov_b32_lit, 8, asm volatile("v_mov_b32 %0, 0x12345" : "=v"(a[i])))
 DEFK(v_or_b32, 8, R2("v_or_b32"))
 DEFK(v_bitop3_b32_sgpr, 8, asm volatile("v_bitop3_b32 %0, %0, %1, %2 bitop3:0x96" : "+v"(a[i]) : "s"(c), "v"(b)))
 DEFK(v_perm_b32_sgpr, 8, asm volatile("v_perm_b32 %0, %0, %1, %2" : "+v"(a[i]) : "v"(b), "s"(c)))
-DEFK(v_cndmask_b32, 8, asm volatile("v_cndmask_b32 %0, %0, %1, vcc" : "+v"(a[i]) : "v"(b)))
 DEFK(v_sub_f32, 8, R2("v_sub_f32"))
 DEFK(v_max_f32, 8, R2("v_max_f32"))
 DEFK(v_lshlrev_b32_inl, 8, asm volatile("v_lshlrev_b32 %0, 5, %0" : "+v"(a[i])))
@@ -156,7 +155,7 @@ int main() {
             E(v_mov_b32_dpp, 8) E(v_mul_f32_lit, 8) E(v_mul_f32_inl, 8) E(v_mul_f32_e64, 8)
             E(v_and_b32_lit, 8) E(v_and_b32_inl, 8) E(v_and_b32_sgpr, 8) E(v_fmac_f32, 8)
             E(v_fmac_f32_lit, 8) E(v_fmamk_f32, 8) E(v_fma_f32_sgpr, 8) E(v_mov_b32_lit, 8)
-            E(v_or_b32, 8) E(v_bitop3_b32_sgpr, 8) E(v_perm_b32_sgpr, 8) E(v_cndmask_b32, 8)
+            E(v_or_b32, 8) E(v_bitop3_b32_sgpr, 8) E(v_perm_b32_sgpr, 8)
             E(v_sub_f32, 8) E(v_max_f32, 8) E(v_lshlrev_b32_inl, 8) E(v_ashrrev_i32, 8)
             E(v_lshl_add_u32, 8) E(v_add3_u32, 8) E(v_alignbit_b32, 8) E(v_max3_f32, 8)
             E(v_rndne_f32, 8) E(v_mul_u32_u24, 8) E(v_cvt_pkrtz_f16_f32, 8) E(v_pk_add_u16, 8)
