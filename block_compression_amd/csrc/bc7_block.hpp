@@ -124,15 +124,16 @@ BCN_HD Moments moments(const uint32_t px[16], uint32_t mask, uint32_t sh = 0) {
   for (int g = 0; g < 4; g++) {
     uint32_t t[4];
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int k = 4 * g + i;
-      t[i] = ((mask >> k) & 1u) ? texel<ROT>(px, k, sh) : 0u;
-    }
-    // (t0.R, t1.R, t0.B, t1.B), (t0.G, t1.G, t0.A, t1.A), then with t2, t3.
+    for (int i = 0; i < 4; i++) t[i] = texel<ROT>(px, 4 * g + i, sh);
+    // (t0.R, t1.R, t0.B, t1.B), (t0.G, t1.G, t0.A, t1.A), then with t2, t3;
+    // the texels outside the mask are cleared in the channel words by one
+    // AND with the group's byte mask (no per-texel selects, whose lane masks
+    // the compiler would otherwise keep live across loops).
+    const uint32_t mb = ((((mask >> (4 * g)) & 0xFu) * 0x00204081u) & 0x01010101u) * 0xFFu;
     const uint32_t a = perm_b32(t[1], t[0], 0x06020400u), b = perm_b32(t[1], t[0], 0x07030501u);
     const uint32_t c = perm_b32(t[3], t[2], 0x06020400u), d = perm_b32(t[3], t[2], 0x07030501u);
-    const uint32_t r = perm_b32(c, a, 0x05040100u), bl = perm_b32(c, a, 0x07060302u);
-    const uint32_t gr = perm_b32(d, b, 0x05040100u);
+    const uint32_t r = perm_b32(c, a, 0x05040100u) & mb, bl = perm_b32(c, a, 0x07060302u) & mb;
+    const uint32_t gr = perm_b32(d, b, 0x05040100u) & mb;
     s[0] = udot4(r, 0x01010101u, s[0]);
     s[1] = udot4(gr, 0x01010101u, s[1]);
     s[2] = udot4(bl, 0x01010101u, s[2]);
@@ -143,7 +144,7 @@ BCN_HD Moments moments(const uint32_t px[16], uint32_t mask, uint32_t sh = 0) {
     q[5] = udot4(gr, bl, q[5]);
     q[7] = udot4(bl, bl, q[7]);
     if (CH == 4) {
-      const uint32_t al = perm_b32(d, b, 0x07060302u);
+      const uint32_t al = perm_b32(d, b, 0x07060302u) & mb;
       s[3] = udot4(al, 0x01010101u, s[3]);
       q[3] = udot4(r, al, q[3]);
       q[6] = udot4(gr, al, q[6]);
@@ -759,21 +760,19 @@ BCN_HD void opt_endpoints(const uint32_t px[16], const uint32_t qblock[2], uint3
   for (int g = 0; g < 4; g++) {
     uint32_t t[4];
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int k = 4 * g + i;
-      t[i] = ((mask >> k) & 1u) ? texel<ROT>(px, k, sh) : 0u;
-    }
+    for (int i = 0; i < 4; i++) t[i] = texel<ROT>(px, 4 * g + i, sh);
+    // Texels outside the mask cleared in the channel words (see moments).
+    const uint32_t mb = ((((mask >> (4 * g)) & 0xFu) * 0x00204081u) & 0x01010101u) * 0xFFu;
     const uint32_t a = perm_b32(t[1], t[0], 0x06020400u), b = perm_b32(t[1], t[0], 0x07030501u);
     const uint32_t c = perm_b32(t[3], t[2], 0x06020400u), d = perm_b32(t[3], t[2], 0x07030501u);
     uint32_t ch[4];
-    ch[0] = perm_b32(c, a, 0x05040100u);
-    ch[1] = perm_b32(d, b, 0x05040100u);
-    ch[2] = perm_b32(c, a, 0x07060302u);
-    ch[3] = CH == 4 ? perm_b32(d, b, 0x07060302u) : 0u;
+    ch[0] = perm_b32(c, a, 0x05040100u) & mb;
+    ch[1] = perm_b32(d, b, 0x05040100u) & mb;
+    ch[2] = perm_b32(c, a, 0x07060302u) & mb;
+    ch[3] = CH == 4 ? perm_b32(d, b, 0x07060302u) & mb : 0u;
     // Index nibbles of the group's texels to bytes, and their mask.
     const uint32_t nib = (qblock[g >> 1] >> (16 * (g & 1))) & 0xFFFFu;
     const uint32_t qb = (nib & 0xFu) | ((nib & 0xF0u) << 4) | ((nib & 0xF00u) << 8) | ((nib & 0xF000u) << 12);
-    const uint32_t mb = ((((mask >> (4 * g)) & 0xFu) * 0x00204081u) & 0x01010101u) * 0xFFu;
     const uint32_t qm = qb & mb, xb = qb ^ (0x01010101u * (uint32_t)L1);
     uq = udot4(qm, 0x01010101u, uq);
     uqq = udot4(qm, qm, uqq);
