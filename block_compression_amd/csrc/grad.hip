@@ -167,9 +167,12 @@ __device__ __forceinline__ uint32_t block_mode(const uint32_t* in, uint32_t b) {
 }
 
 // Sum over the four lanes of a quad, (l0 + l1) + (l2 + l3) in every lane.
+// The lane moves are written as update_dpp with all rows and banks enabled
+// and bound_ctrl set, which lets the compiler fold each into its add
+// (v_add_f32 with a DPP source: one instruction per step instead of two).
 __device__ __forceinline__ float quad_sum(float v) {
-  float a = v + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
-  return a + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(a), 0x4E, 0xF, 0xF, false));
+  float a = v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, true));
+  return a + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(a), 0x4E, 0xF, 0xF, true));
 }
 
 __global__ void refine_clear_kernel(uint32_t* ws) {
