@@ -41,13 +41,21 @@ __global__ __launch_bounds__(kDecGroup) void decode_rgba8_kernel(const uint8_t* 
   uint32_t in[4];
 #pragma unroll
   for (int i = 0; i < 4; i++) in[i] = i < BB / 4 ? src[i] : 0u;
-  uint32_t px[16];
-  // BC7's three subset records per lane, interleaved by lane (12 KiB).
-  __shared__ Bc7SubsetData subs[FMT == BCN_BC7 ? 3 * kDecGroup : 1];
-  decode_rgba8_block<FMT>(in, px, subs + (FMT == BCN_BC7 ? threadIdx.x : 0), kDecGroup);
   uint32_t yy, xx;
   block_pos(rows, idx, yy, xx);
   uint8_t* o = rgba + (size_t)yy * 4 * pitch + (size_t)xx * 16;
+  // BC7's three subset records per lane, interleaved by lane (12 KiB).
+  static_assert(kDecGroup == (int)kBc7SubRows, "the BC7 record addresses assume 256 lanes");
+  __shared__ Bc7SubsetData subs[FMT == BCN_BC7 ? 3 * kDecGroup : 1];
+  if constexpr (FMT == BCN_BC7) {
+    // Each row is stored as soon as it is decoded.
+    decode_bc7_rows(in, subs, threadIdx.x, [&](int y, uint32_t a, uint32_t b, uint32_t c, uint32_t e) {
+      *reinterpret_cast<uint4*>(o + (size_t)y * pitch) = make_uint4(a, b, c, e);
+    });
+    return;
+  }
+  uint32_t px[16];
+  decode_rgba8_block<FMT>(in, px, subs, threadIdx.x);
 #pragma unroll
   for (int y = 0; y < 4; y++)
     *reinterpret_cast<uint4*>(o + (size_t)y * pitch) =

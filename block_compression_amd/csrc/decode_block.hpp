@@ -7,17 +7,19 @@
 // statement of the bit layout, and the tests check both against the
 // reference's routines.
 #pragma once
+#include <cstddef>
+
 #include "bc6h_block.hpp"
 #include "bcn_common.hpp"
 
 namespace bcn {
 
-// (hi:lo) >> n, low 32 bits, for n in 0..31 (v_alignbit_b32 on the GPU).
+// (hi:lo) >> n, low 32 bits, for n modulo 32 (v_alignbit_b32 on the GPU).
 BCN_HD uint32_t funnel_shr(uint32_t hi, uint32_t lo, uint32_t n) {
 #if defined(__HIP_DEVICE_COMPILE__)
   return __builtin_amdgcn_alignbit(hi, lo, n);
 #else
-  return (uint32_t)((((uint64_t)hi << 32) | lo) >> n);
+  return (uint32_t)((((uint64_t)hi << 32) | lo) >> (n & 31u));
 #endif
 }
 
@@ -30,6 +32,7 @@ struct BitReader {
   uint32_t w0, w1, w2, w3;
   BCN_HD BitReader(const uint32_t w[4]) : w0(w[0]), w1(w[1]), w2(w[2]), w3(w[3]) {}
   BCN_HD uint32_t read(uint32_t n) {
+    n &= 31u;  // the shift instructions' own rule: a caller may pass a word whose low bits are n
     uint32_t v = w0 & ((1u << n) - 1u);
     w0 = funnel_shr(w1, w0, n);
     w1 = funnel_shr(w2, w1, n);
@@ -159,137 +162,303 @@ struct alignas(16) Bc7SubsetData {
   uint32_t crg, cba, drg, dba;
 };
 
+// The record of subset s for lane l (l < 256) is element s * kBc7SubRows + l
+// of the record array: byte address s * 4096 + 16 l, whose low byte comes
+// from the lane alone and whose second byte is s << 4 | l >> 4.
+constexpr uint32_t kBc7SubRows = 256;
+
+// Per-block constants of the BC7 decode, one 64-byte row per form:
+//   rows 0-3: modes 0-3; row 8: a first byte of 0 (every constant 0, so the
+//   block decodes to transparent black); rows 9-16: mode 4 by rotation and
+//   index selector; rows 17-20: mode 5 by rotation; rows 21, 22: modes 6, 7.
+// A block's row is mode + kBc7Adj[mode] + (pr & kBc7Vmask[mode]), pr being
+// the bits that follow the mode bits (the partition, or the rotation and
+// index selector).
+struct alignas(16) Bc7DecForm {
+  // Endpoint expansion in 16-bit lanes (R, G) and (B, A).  A field v of w
+  // bits with p-bit p (t = w + 1 bits) or without (t = w) expands as
+  // expand_bits (block.rs:1159-1163) to x (2^8 + 2^(8-t)) >> t for x = 2v + p
+  // or v (the two bit groups of the expansion do not overlap, so the OR is a
+  // sum): v * mul + p * k, then >> bt.  c_ba adds 255 << 8 to the alpha lane
+  // of modes without alpha (alpha 255).
+  uint32_t mul_rg, mul_ba, k_rg, k_ba;
+  // lens bytes: header bits, colour chunk bits (ne fields), alpha chunk bits,
+  // alpha chunk + p-bits.  wid bytes: colour and alpha field bits, index bits
+  // of set 1 and set 2.
+  uint32_t c_ba, bt_rg, bt_ba, lens;
+  // kk: the 4095 / (2^bits - 1) weight multipliers of the colour (low) and
+  // alpha (high) lane; ipsel: v_perm selector that places the set 1 and set
+  // 2 indices in those lanes (the index selector of mode 4 swaps them);
+  // out_sel: selector packing the texel from the lanes' high bytes, with the
+  // rotated channel and alpha exchanged (block.rs:1228-1240).
+  uint32_t wid, kk, ipsel, out_sel;
+  // prow: first row of the block's partition kind in kBc7Dec.part (the
+  // decoder takes it from kBc7Prow16; a first byte of 0 then reads mode 0's
+  // rows, harmless with every constant 0); s2sh:
+  // where index set 2 starts in the block's top 64 bits; ins: the insertion
+  // mask of texel 0's missing index bit in set 1 (bits 0-7) and set 2 (bits
+  // 16-31), and in bits 8-15 the shift from endpoint to p-bit number (1 for
+  // the shared p-bits of mode 1).
+  uint32_t prow, s2sh, ins, pad;
+};
+
+// Per partition kind (one subset; two or three subsets at 2 or 3 index bits)
+// and partition: the subset of texel 4 q + t in byte t of sub4[q] (times 16,
+// the record address's second byte), and the masks ~((1 << P) - 1) (64 bits,
+// low word first) that insert the anchors' missing index bits at their
+// positions P = (a + 1) ib - 1 of the set 1 stream, in ascending order (0
+// inserts nothing; insert_zero64).
+struct alignas(16) Bc7DecPart {
+  uint32_t sub4[4];
+  uint32_t ins[4];
+};
+
+struct Bc7DecTables {
+  Bc7DecForm form[23];
+  Bc7DecPart part[272];  // 1 subset (rows 0-15), 2 subsets ib 2 (16-79), ib 3 (80-143), 3 subsets ib 2 (144-207), ib 3 (208-271)
+};
+
+constexpr uint32_t kBc7Pbw = 0x60236664u;  // bits after the mode bits: pb + rb + isb, by mode (nibbles)
+constexpr uint32_t kBc7Adj = 0xFFC50000u;  // row - mode
+constexpr uint32_t kBc7Vmask = 0x00370000u;  // pr bits that select the row (modes 4, 5)
+constexpr uint32_t kBc7Prow16 = 0x1000195Du;  // first partition row / 16, by mode (a first byte of 0 reads mode 0's)
+
+constexpr uint32_t bc7_part_row(int ns, int ib) {
+  return ns == 1 ? 0u : (ns == 2 ? (ib == 2 ? 16u : 80u) : (ib == 2 ? 144u : 208u));
+}
+
+// ~((1 << p) - 1) as two words; p = 64 gives 0.
+constexpr void bc7_ins_mask(int p, uint32_t& lo, uint32_t& hi) {
+  const uint64_t m = p >= 64 ? ~0ull : (1ull << p) - 1ull;
+  lo = (uint32_t)~m;
+  hi = (uint32_t)(~m >> 32);
+}
+
+constexpr Bc7DecForm bc7_dec_form(int m, int rot, int isb) {
+  constexpr Bc7ModeInfo t[8] = {{3, 4, 0, 0, 4, 0, 1, 3, 0}, {2, 6, 0, 0, 6, 0, 2, 3, 0}, {3, 6, 0, 0, 5, 0, 0, 2, 0},
+                                {2, 6, 0, 0, 7, 0, 1, 2, 0}, {1, 0, 2, 1, 5, 6, 0, 2, 3}, {1, 0, 2, 0, 7, 8, 0, 2, 2},
+                                {1, 0, 0, 0, 7, 7, 1, 4, 0}, {2, 6, 0, 0, 5, 5, 1, 2, 0}};
+  const Bc7ModeInfo& i = t[m];
+  Bc7DecForm f{};
+  const int ne = 2 * i.ns, hp = i.pbit != 0 ? 1 : 0, ap = i.pbit == 1 ? 1 : 0;
+  const int tc = i.cb + hp, ta = i.ab ? i.ab + ap : 8;
+  const uint32_t mc = 256u + (1u << (8 - tc)), ma = 256u + (1u << (8 - ta));
+  const uint32_t mul_c = mc << hp, k_c = hp ? mc : 0u;
+  const uint32_t mul_a = i.ab ? ma << ap : 0u, k_a = (i.ab && ap) ? ma : 0u;
+  f.mul_rg = mul_c * 0x10001u;
+  f.mul_ba = mul_c | (mul_a << 16);
+  f.k_rg = k_c * 0x10001u;
+  f.k_ba = k_c | (k_a << 16);
+  f.c_ba = i.ab ? 0u : 0xFF000000u;
+  f.bt_rg = (uint32_t)tc * 0x10001u;
+  f.bt_ba = (uint32_t)tc | ((uint32_t)ta << 16);
+  const int np = i.pbit == 1 ? ne : (i.pbit == 2 ? 2 : 0);
+  const int h = m + 1 + i.pb + i.rb + i.isb;
+  f.lens = (uint32_t)h | ((uint32_t)(ne * i.cb) << 8) | ((uint32_t)(ne * i.ab) << 16) |
+           ((uint32_t)(ne * i.ab + np) << 24);
+  f.wid = (uint32_t)i.cb | ((uint32_t)i.ab << 8) | ((uint32_t)i.ib << 16) | ((uint32_t)i.ib2 << 24);
+  const uint32_t k1 = i.ib == 2 ? 1365u : (i.ib == 3 ? 585u : 273u);
+  const uint32_t k2 = i.ib2 == 2 ? 1365u : (i.ib2 == 3 ? 585u : 273u);
+  f.kk = i.ib2 == 0 ? k1 * 0x10001u : (isb ? k2 | (k1 << 16) : k1 | (k2 << 16));
+  f.ipsel = i.ib2 == 0 ? 0x0C000C00u : (isb ? 0x0C000C04u : 0x0C040C00u);
+  f.out_sel = rot == 1 ? 0x01050307u : (rot == 2 ? 0x03050701u : (rot == 3 ? 0x05070301u : 0x07050301u));
+  f.prow = bc7_part_row(i.ns, i.ib);
+  f.s2sh = i.ib2 ? (uint32_t)(65 - 16 * i.ib2) : 0u;
+  f.ins = ((1u << (i.ib - 1)) - 1u) | ((i.pbit == 2 ? 1u : 0u) << 8) | (i.ib2 ? ((1u << (i.ib2 - 1)) - 1u) << 16 : 0u);
+  return f;
+}
+
+constexpr Bc7DecTables make_bc7_dec_tables() {
+  constexpr PartitionTables pt = make_partition_tables();
+  Bc7DecTables d{};
+  for (int m = 0; m < 8; m++) {
+    const int base = m + (int)((kBc7Adj >> (4 * m)) & 15u), nv = m == 4 ? 8 : (m == 5 ? 4 : 1);
+    for (int v = 0; v < nv; v++) d.form[base + v] = bc7_dec_form(m, m == 4 ? (v & 3) : (m == 5 ? v : 0), m == 4 ? v >> 2 : 0);
+  }
+  d.form[8] = Bc7DecForm{};  // a first byte of 0
+  d.form[8].out_sel = 0x07050301u;
+  for (int kind = 0; kind < 4; kind++) {
+    const int ns = kind < 2 ? 2 : 3, ib = (kind & 1) ? 3 : 2;
+    for (int p = 0; p < 64; p++) {
+      Bc7DecPart& r = d.part[16 + 64 * kind + p];
+      const int pi = ns == 3 ? 64 + p : p;
+      for (int k = 0; k < 16; k++) r.sub4[k >> 2] |= ((kShapes[pi] >> (2 * k)) & 3u) << (8 * (k & 3) + 4);
+      int a = (pt.anchor1[pi] + 1) * ib - 1, b = ns == 3 ? (pt.anchor2[pi] + 1) * ib - 1 : 64;
+      if (b < a) { const int s = a; a = b; b = s; }
+      bc7_ins_mask(a, r.ins[0], r.ins[1]);
+      bc7_ins_mask(b, r.ins[2], r.ins[3]);
+    }
+  }
+  return d;
+}
+
+BCN_TABLE Bc7DecTables kBc7Dec = make_bc7_dec_tables();
+
+constexpr bool bc7_prow16_holds() {
+  constexpr Bc7DecTables d = make_bc7_dec_tables();
+  for (int m = 0; m < 8; m++)
+    if (d.form[m + (int)((kBc7Adj >> (4 * m)) & 15u)].prow != ((kBc7Prow16 >> (4 * m)) & 15u) << 4) return false;
+  return true;
+}
+static_assert(bc7_prow16_holds(), "kBc7Prow16 disagrees with the form rows");
+
+// All ones if bit `off` (modulo 32) of x is set, else 0 (v_bfe_i32).
+BCN_HD uint32_t bit_mask(uint32_t x, uint32_t off) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return (uint32_t)__builtin_amdgcn_sbfe((int)x, off, 1);
+#else
+  return 0u - ((x >> (off & 31u)) & 1u);
+#endif
+}
+
+// x with a 0 bit inserted at bit P, for im = ~((1 << P) - 1): the bits below
+// P stay and those from P move up one, (x & ~im) + 2 (x & im) = x + (x & im).
+BCN_HD uint64_t insert_zero64(uint64_t x, uint64_t im) { return x + (x & im); }
+
 // decode_block_bc7 (block.rs:801-1252) for one block, every mode in one
-// branch-free path.  `sub` holds this lane's three subset records at stride
-// `stride` (LDS rows on the GPU), read back by subset id per texel.
-//   * The mode's field widths come from nibble tables; a field the mode does
-//     not have is a read of width 0.
-//   * The mode bits, partition, rotation and index selector are one read; the
-//     six endpoints of a channel (ne * width <= 30 bits) are one read split
-//     by v_bfe_u32; so are the alpha endpoints and the p-bits.
-//   * Index set 1 is read from the two low words of what remains; index set 2
-//     (modes 4 and 5) from the top of the block, where it ends.  Anchor texels
-//     read one bit less, from a per-block anchor mask.
+// branch-free path.  `sub` is the record array of the workgroup (or of the
+// caller, with lane 0): this lane's subset records are written and read back
+// at the addresses kBc7SubRows describes.
+//   * Per-block constants come from one row of kBc7Dec.form, chosen by the
+//     mode and, for modes 4 and 5, the rotation and index selector; the
+//     partition's texel subsets and anchor positions from one row of
+//     kBc7Dec.part.
+//   * The header, then the colour chunks (each channel's ne fields), are
+//     consumed by four skips of a 128-bit reader; the alpha chunk, the p-bits
+//     and index set 1 follow within the next 96 bits.  Index set 2 (modes 4
+//     and 5) ends the block.
+//   * Each index set is made uniform before the texel loop: a 0 bit is
+//     inserted at every anchor index's missing top bit, so index k sits at bit
+//     k * bits (k < 8) of the stream or (k - 8) * bits of the stream shifted by
+//     8 * bits -- one v_bfe_u32 per index.
 //   * Weights are round(64 i / (2^bits - 1)) = (i * (4095 / m) + 32) >> 6 in
 //     16-bit lanes: the colour weight in the low lane, the alpha weight in the
-//     high lane (set 1 in both without a set 2; the index selector swaps the
-//     sets).  R and G take the low lane, B the low and A the high lane.
-//   * The rotation (block.rs:1228-1240) exchanges a colour channel and alpha
-//     after interpolation: it only changes the byte selector that packs the
-//     texel.
-BCN_HD void decode_bc7(const uint32_t in[4], uint32_t px[16], Bc7SubsetData* sub, int stride) {
-  // A first byte of 0 is no mode: the block decodes as transparent black.
-  // It takes mode 7's layout here and a byte selector of zeros at the end,
-  // so every lane runs the same path.
-  const uint32_t first = in[0] & 0xFFu;
-  const uint32_t mode = (uint32_t)__builtin_ctz(first | 0x80u), m4 = 4u * mode;
-  constexpr uint32_t kNs = bc7_nibbles(&Bc7ModeInfo::ns), kPb = bc7_nibbles(&Bc7ModeInfo::pb),
-                     kRb = bc7_nibbles(&Bc7ModeInfo::rb), kIsb = bc7_nibbles(&Bc7ModeInfo::isb),
-                     kCb = bc7_nibbles(&Bc7ModeInfo::cb), kAb = bc7_nibbles(&Bc7ModeInfo::ab),
-                     kPk = bc7_nibbles(&Bc7ModeInfo::pbit), kIb = bc7_nibbles(&Bc7ModeInfo::ib),
-                     kIb2 = bc7_nibbles(&Bc7ModeInfo::ib2);
-  const uint32_t ns = bfe_u32(kNs, m4, 4), pb = bfe_u32(kPb, m4, 4), rb = bfe_u32(kRb, m4, 4),
-                 isbits = bfe_u32(kIsb, m4, 4), cb = bfe_u32(kCb, m4, 4), ab = bfe_u32(kAb, m4, 4),
-                 pk = bfe_u32(kPk, m4, 4), ib = bfe_u32(kIb, m4, 4), ib2 = bfe_u32(kIb2, m4, 4);
+//     high lane.  R and G take the low lane, B the low and A the high lane.
+//   * A texel's record address is one v_perm_b32 of the partition row's byte
+//     and the lane's base.
+//   * Each row of four texels goes to `emit(y, t0, t1, t2, t3)` as soon as it
+//     is decoded (the image kernel stores it then).
+template <typename RowSink>
+BCN_HD void decode_bc7_rows(const uint32_t in[4], Bc7SubsetData* sub, uint32_t lane, RowSink&& emit) {
+  const uint32_t mode = (uint32_t)__builtin_ctz(in[0] | 0x100u), m4 = 4u * mode;
+  const uint32_t pr = bfe_u32(in[0], mode + 1u, bfe_u32(kBc7Pbw, m4, 4));
+  const uint32_t row = mode + bfe_u32(kBc7Adj, m4, 4) + (pr & bfe_u32(kBc7Vmask, m4, 4));
+  // Both rows through one 32-bit byte offset from the table's base, each
+  // from the mode bits alone, so the two loads go out together.
+  const uint32_t prow = bfe_u32(kBc7Prow16, m4, 4) << 4;
+  const char* tab = reinterpret_cast<const char*>(&kBc7Dec);
+  const Bc7DecForm f = *reinterpret_cast<const Bc7DecForm*>(tab + row * (uint32_t)sizeof(Bc7DecForm));
+  const Bc7DecPart pt = *reinterpret_cast<const Bc7DecPart*>(
+      tab + offsetof(Bc7DecTables, part) + (prow + pr) * (uint32_t)sizeof(Bc7DecPart));
+  // Chunks: header, then R, G, B (L bits each), then alpha, p-bits, set 1.
   BitReader br(in);
-  const uint32_t h0 = mode + 1, h1 = h0 + pb, h2 = h1 + rb;
-  const uint32_t hdr = br.read(h2 + isbits);
-  const uint32_t part = bfe_u32(hdr, h0, pb), rotation = bfe_u32(hdr, h1, rb), isb = bfe_u32(hdr, h2, isbits);
-  const uint32_t ne = 2 * ns;
-  // Endpoint fields: per channel one read of ne * width bits (<= 30).  An
-  // endpoint the mode lacks reads above the chunk and gets 0.
-  int ep[6][4];
-#pragma unroll
-  for (int c = 0; c < 4; c++) {
-    const uint32_t w = c < 3 ? cb : ab;
-    const uint32_t chunk = br.read(ne * w);
-#pragma unroll
-    for (int e = 0; e < 6; e++) ep[e][c] = (int)bfe_u32(chunk, (uint32_t)e * w, w);
-  }
-  // P-bits: ne of them (kind 1, one per endpoint) or 2 (kind 2, one per
-  // subset shared by its two endpoints, spread here to one per endpoint),
-  // else none.
-  const uint32_t praw = br.read(pk == 1 ? ne : (pk == 2 ? 2u : 0u));
-  const uint32_t pch = pk == 2 ? (praw & 1u) * 3u + (praw & 2u) * 6u : praw;
-  const uint32_t has_p = pk != 0 ? 1u : 0u, alpha_p = pk == 1 ? 1u : 0u;
-  const uint32_t cbits = cb + has_p, abits = ab + alpha_p;
-  // Each endpoint as two words of 16-bit lanes, (R | G << 16) and
-  // (B | A << 16).  A field v of w bits with p-bit p (t = w + 1 bits) or
-  // without (t = w) is placed at the top of 8 bits by one multiply-add,
-  //   (2 v + p) << (8 - t) = v * 2^(8 - w) + p * 2^(7 - w),
-  // and expand_bits (block.rs:1159-1163) is then x | (x >> t).  A mode
-  // without alpha decodes alpha 255 (expand of 0 ORed with 255).
-  const uint32_t pc = has_p ? 1u << (7 - cb) : 0u, pa = alpha_p ? 1u << (7 - ab) : 0u;
-  const uint32_t mul_rg = (1u << (8 - cb)) * 0x10001u, mul_ba = (1u << (8 - cb)) | ((1u << (8 - ab)) << 16);
-  const uint32_t k_rg = pc * 0x10001u, k_ba = pc | (pa << 16);
-  const uint32_t bt_rg = cbits * 0x10001u, bt_ba = cbits | ((ab ? abits : 8u) << 16);
-  const uint32_t alpha_or = ab ? 0u : 0x00FF0000u;
+  br.skip(f.lens);
+  const uint32_t L = f.lens >> 8, La = f.lens >> 16;
+  const uint32_t chr = br.w0;
+  br.skip(L);
+  const uint32_t chg = br.w0;
+  br.skip(L);
+  const uint32_t chb = br.w0;
+  br.skip(L);
+  const uint32_t cha = br.w0;
+  const uint32_t pbits = funnel_shr(br.w1, br.w0, La);
+  const uint32_t s1o = f.lens >> 24;
+  uint64_t s1 = ((uint64_t)funnel_shr(br.w2, br.w1, s1o) << 32) | funnel_shr(br.w1, br.w0, s1o);
+  uint64_t s2 = (((uint64_t)in[3] << 32) | in[2]) >> (f.s2sh & 63u);
+  // Texel 0's missing bit in both sets, then the anchors of set 1.
+  s1 = insert_zero64(s1, ~(uint64_t)(f.ins & 0xFFu));
+  s1 = insert_zero64(s1, ((uint64_t)pt.ins[1] << 32) | pt.ins[0]);
+  s1 = insert_zero64(s1, ((uint64_t)pt.ins[3] << 32) | pt.ins[2]);
+  s2 = insert_zero64(s2, ~(uint64_t)(f.ins >> 16));
+  // Endpoints: field e of a channel at bit e * width of its chunk.
+  const uint32_t cbx = f.wid, abx = f.wid >> 8, ibx = f.wid >> 16, ib2x = f.wid >> 24;
+  const uint32_t pks = f.ins >> 8;
+  // Field offsets e * width as running sums (in_vgpr keeps them additions:
+  // the compiler would otherwise fold them into slower multiplies and shifts).
+  uint32_t oc = 0u, oa = 0u;
+  // 256 in both lanes, in a register: with the constant in view the compiler
+  // splits each multiply-add into a shift and an OR.
+  const uint32_t k256 = in_vgpr(0x01000100u);
+  // Subsets 1 and 2 only where some block of the wave has them.
+  const bool two = wave_any(prow >= 16u), three = wave_any(prow >= 144u);
 #pragma unroll
   for (int sb = 0; sb < 3; sb++) {
+    if ((sb == 1 && !two) || (sb == 2 && !three)) continue;
     uint32_t rg[2], ba[2];
 #pragma unroll
     for (int h = 0; h < 2; h++) {
-      const int e = 2 * sb + h;
-      const uint32_t p = bfe_u32(pch, (uint32_t)e, 1);
-      const uint32_t x = pk_mad16((uint32_t)ep[e][0] | ((uint32_t)ep[e][1] << 16), mul_rg, p * k_rg);
-      const uint32_t y = pk_mad16((uint32_t)ep[e][2] | ((uint32_t)ep[e][3] << 16), mul_ba, p * k_ba);
-      rg[h] = x | pk_shr16(x, bt_rg);
-      ba[h] = (y | pk_shr16(y, bt_ba)) | alpha_or;
+      const uint32_t e = 2u * (uint32_t)sb + (uint32_t)h;
+      if (e > 0) {
+        oc = in_vgpr(oc + cbx);
+        oa = in_vgpr(oa + abx);
+      }
+      const uint32_t vrg = bfe_u32(chr, oc, cbx) | (bfe_u32(chg, oc, cbx) << 16);
+      const uint32_t vba = bfe_u32(chb, oc, cbx) | (bfe_u32(cha, oa, abx) << 16);
+      const uint32_t pm = bit_mask(pbits, e >> (pks & 31u));
+      rg[h] = pk_shr16(pk_mad16(vrg, f.mul_rg, pm & f.k_rg), f.bt_rg);
+      ba[h] = pk_shr16(pk_mad16(vba, f.mul_ba, (pm & f.k_ba) | f.c_ba), f.bt_ba);
     }
     Bc7SubsetData d;
-    d.crg = pk_mad16(rg[0], 0x01000100u, 0x00800080u);
-    d.cba = pk_mad16(ba[0], 0x01000100u, 0x00800080u);
+    d.crg = pk_mad16(rg[0], k256, 0x00800080u);
+    d.cba = pk_mad16(ba[0], k256, 0x00800080u);
     d.drg = pk_shl16(pk_sub16(rg[1], rg[0]), 0x00020002u);
     d.dba = pk_shl16(pk_sub16(ba[1], ba[0]), 0x00020002u);
-    sub[sb * stride] = d;
+    sub[(uint32_t)sb * kBc7SubRows + lane] = d;
   }
-  // Weight lanes: perm_b32(x2, x1, ipsel) places the set 1 and set 2 indices
-  // x1, x2 in the colour (low) and alpha (high) lanes, and kk the matching
-  // 4095 / m multipliers.
-  const uint32_t m1 = (uint32_t)bptc_weight_mul((int)ib), m2 = (uint32_t)bptc_weight_mul((int)ib2);
-  const uint32_t ipsel = ib2 == 0 ? 0x0C000C00u : (isb ? 0x0C000C04u : 0x0C040C00u);
-  const uint32_t kk = ib2 == 0 ? m1 * 0x10001u : (isb ? m2 | (m1 << 16) : m1 | (m2 << 16));
-  // Texel bytes: the high byte of each lane, R, G from the (R, G) word
-  // (bytes 1, 3) and B, A from the (B, A) word (bytes 5, 7), with the
-  // rotated channel and alpha exchanged.
-  const uint32_t out_sel = first == 0 ? 0x0C0C0C0Cu
-                         : (rotation == 1 ? 0x01050307u
-                            : (rotation == 2 ? 0x03050701u : (rotation == 3 ? 0x05070301u : 0x07050301u)));
-  const uint32_t pidx = ns == 3 ? 64u + part : part;
-  const uint32_t pattern = ns == 1 ? 0u : kPart.subset[pidx];
-  const uint32_t a1 = ns >= 2 ? kPart.anchor1[pidx] : 0u, a2 = ns == 3 ? kPart.anchor2[pidx] : 0u;
-  const uint32_t amask = 1u | (1u << a1) | (1u << a2);  // anchors of set 1
-  // Index set 1 sits at the bottom of what remains (at most 63 bits, in the
-  // two low words); index set 2 ends the block: T2 = 16 ib2 - 1 bits.
-  uint32_t i0 = br.w0, i1 = br.w1;
-  const uint32_t t2 = ib2 ? 16u * ib2 - 1u : 0u, sh = 64u - t2;  // 17, 33 or 64
-  uint32_t j0 = sh < 32 ? funnel_shr(in[3], in[2], sh) : (sh < 64 ? in[3] >> (sh - 32) : 0u);
-  uint32_t j1 = sh < 32 ? in[3] >> sh : 0u;
+  // Index k < 8 at bit k * bits of the stream, k >= 8 at (k - 8) * bits of
+  // the stream shifted by 8 * bits.
+  const uint32_t s1lo = (uint32_t)s1, s1mid = (uint32_t)(s1 >> ((ibx << 3) & 63u));
+  const uint32_t s2lo = (uint32_t)s2, s2mid = (uint32_t)(s2 >> ((ib2x << 3) & 63u));
+  const uint32_t lanehi = (lane >> 4) * 0x01010101u, laneb = lane << 4;
+  const char* base = reinterpret_cast<const char*>(sub);
+  uint32_t o1[8], o2[8];
+  o1[0] = 0u;
+  o2[0] = 0u;
 #pragma unroll
-  for (int k = 0; k < 16; k++) {
-    const uint32_t n1 = ib - bfe_u32(amask, (uint32_t)k, 1);
-    const uint32_t n2 = k == 0 ? (ib2 ? ib2 - 1u : 0u) : ib2;
-    const uint32_t x1 = bfe_u32(i0, 0, n1), x2 = bfe_u32(j0, 0, n2);
-    i0 = funnel_shr(i1, i0, n1);
-    i1 >>= n1;
-    j0 = funnel_shr(j1, j0, n2);
-    j1 >>= n2;
-    // Both weights: i * (4095 / m) + 32 <= 4127 per lane, then >> 6.
-    const uint32_t wp = pk_mad_shr6(perm_b32(x2, x1, ipsel), kk, 0x00200020u);
-    const uint32_t s = bfe_u32(pattern, 2u * (uint32_t)k, 2);
-    const Bc7SubsetData d = sub[(int)s * stride];
-    const uint32_t rg = pk_mad16_lo(d.drg, wp, d.crg);
-    const uint32_t ba = pk_mad16(d.dba, wp, d.cba);
-    px[k] = perm_b32(ba, rg, out_sel);
+  for (int j = 1; j < 8; j++) {
+    o1[j] = in_vgpr(o1[j - 1] + ibx);
+    o2[j] = in_vgpr(o2[j - 1] + ib2x);
   }
+  // A row at a time: its four record reads go out together.
+#pragma unroll
+  for (int y = 0; y < 4; y++) {
+    const uint32_t row4 = pt.sub4[y] | lanehi;
+    Bc7SubsetData d[4];
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+      d[x] = *reinterpret_cast<const Bc7SubsetData*>(base + perm_b32(row4, laneb, 0x0C0C0400u + ((uint32_t)x << 8)));
+    uint32_t px[4];
+#pragma unroll
+    for (int x = 0; x < 4; x++) {
+      const int k = 4 * y + x;
+      const uint32_t x1 = bfe_u32(k < 8 ? s1lo : s1mid, o1[k & 7], ibx);
+      const uint32_t x2 = bfe_u32(k < 8 ? s2lo : s2mid, o2[k & 7], ib2x);
+      // Both weights: i * (4095 / m) + 32 <= 4127 per lane, then >> 6.
+      const uint32_t wp = pk_mad_shr6(perm_b32(x2, x1, f.ipsel), f.kk, 0x00200020u);
+      const uint32_t rg = pk_mad16_lo(d[x].drg, wp, d[x].crg);
+      const uint32_t ba = pk_mad16(d[x].dba, wp, d[x].cba);
+      px[x] = perm_b32(ba, rg, f.out_sel);
+    }
+    emit(y, px[0], px[1], px[2], px[3]);
+  }
+}
+
+// decode_bc7_rows into px[16].
+BCN_HD void decode_bc7(const uint32_t in[4], uint32_t px[16], Bc7SubsetData* sub, uint32_t lane) {
+  decode_bc7_rows(in, sub, lane, [&](int y, uint32_t a, uint32_t b, uint32_t c, uint32_t e) {
+    px[4 * y] = a;
+    px[4 * y + 1] = b;
+    px[4 * y + 2] = c;
+    px[4 * y + 3] = e;
+  });
 }
 
 // The RGBA8 texels of one block of format FMT (bcn_variant BC1..BC5, BC7) as
 // decompress_blocks_as_rgba8 writes them (src/decode.rs:336-387): BC4 ->
 // (R, 0, 0, 0), BC5 -> (R, G, 0, 0).  `in` holds the block's 8 or 16 bytes;
-// `sub`/`stride` are BC7's subset records (decode_bc7).
+// `sub`/`lane` are BC7's subset records (decode_bc7).
 template <int FMT>
-BCN_HD void decode_rgba8_block(const uint32_t in[4], uint32_t px[16], Bc7SubsetData* sub, int stride) {
+BCN_HD void decode_rgba8_block(const uint32_t in[4], uint32_t px[16], Bc7SubsetData* sub, uint32_t lane) {
   if (FMT == 0) {  // BC1
     decode_color(in[0], in[1], false, px);
   } else if (FMT == 1 || FMT == 2) {  // BC2, BC3: colour, then alpha in byte 3
@@ -319,7 +488,7 @@ BCN_HD void decode_rgba8_block(const uint32_t in[4], uint32_t px[16], Bc7SubsetD
 #pragma unroll
     for (int k = 0; k < 16; k++) px[k] = perm_b32(g4[k >> 2], r4[k >> 2], 0x0C0C0400u | ((k & 3) * 0x101u));
   } else {  // BC7
-    decode_bc7(in, px, sub, stride);
+    decode_bc7(in, px, sub, lane);
   }
 }
 

@@ -151,14 +151,15 @@ extern "C" int kcheck_decode(int fmt, int is_signed, const uint8_t* blocks, size
       continue;
     }
     uint32_t px[16];
-    bcn::Bc7SubsetData sub[3];
+    // The BC7 decoder's record array, as one lane (0) of a 256-lane group.
+    static bcn::Bc7SubsetData sub[3 * bcn::kBc7SubRows];
     switch (fmt) {
-      case 0: bcn::decode_rgba8_block<0>(in, px, sub, 1); break;
-      case 1: bcn::decode_rgba8_block<1>(in, px, sub, 1); break;
-      case 2: bcn::decode_rgba8_block<2>(in, px, sub, 1); break;
-      case 3: bcn::decode_rgba8_block<3>(in, px, sub, 1); break;
-      case 4: bcn::decode_rgba8_block<4>(in, px, sub, 1); break;
-      default: bcn::decode_rgba8_block<6>(in, px, sub, 1); break;
+      case 0: bcn::decode_rgba8_block<0>(in, px, sub, 0); break;
+      case 1: bcn::decode_rgba8_block<1>(in, px, sub, 0); break;
+      case 2: bcn::decode_rgba8_block<2>(in, px, sub, 0); break;
+      case 3: bcn::decode_rgba8_block<3>(in, px, sub, 0); break;
+      case 4: bcn::decode_rgba8_block<4>(in, px, sub, 0); break;
+      default: bcn::decode_rgba8_block<6>(in, px, sub, 0); break;
     }
     std::memcpy((uint32_t*)out + i * 16, px, sizeof(px));
   }
