@@ -194,12 +194,11 @@ struct alignas(16) Bc7DecForm {
   uint32_t wid, kk, ipsel, out_sel;
   // prow: first row of the block's partition kind in kBc7Dec.part (the
   // decoder takes it from kBc7Prow16; a first byte of 0 then reads mode 0's
-  // rows, harmless with every constant 0); s2sh:
-  // where index set 2 starts in the block's top 64 bits; ins: the insertion
-  // mask of texel 0's missing index bit in set 1 (bits 0-7) and set 2 (bits
-  // 16-31), and in bits 8-15 the shift from endpoint to p-bit number (1 for
-  // the shared p-bits of mode 1).
-  uint32_t prow, s2sh, ins, pad;
+  // rows, harmless with every constant 0); s2sh: where index set 2 starts in
+  // the block's top 64 bits; ins1, ins2: low words of the insertion masks
+  // (insert_zero64) of texel 0's missing index bit in set 1 and set 2 (their
+  // high words are all ones).
+  uint32_t prow, s2sh, ins1, ins2;
 };
 
 // Per partition kind (one subset; two or three subsets at 2 or 3 index bits)
@@ -264,7 +263,8 @@ constexpr Bc7DecForm bc7_dec_form(int m, int rot, int isb) {
   f.out_sel = rot == 1 ? 0x01050307u : (rot == 2 ? 0x03050701u : (rot == 3 ? 0x05070301u : 0x07050301u));
   f.prow = bc7_part_row(i.ns, i.ib);
   f.s2sh = i.ib2 ? (uint32_t)(65 - 16 * i.ib2) : 0u;
-  f.ins = ((1u << (i.ib - 1)) - 1u) | ((i.pbit == 2 ? 1u : 0u) << 8) | (i.ib2 ? ((1u << (i.ib2 - 1)) - 1u) << 16 : 0u);
+  f.ins1 = ~((1u << (i.ib - 1)) - 1u);
+  f.ins2 = i.ib2 ? ~((1u << (i.ib2 - 1)) - 1u) : 0u;
   return f;
 }
 
@@ -366,13 +366,14 @@ BCN_HD void decode_bc7_rows(const uint32_t in[4], Bc7SubsetData* sub, uint32_t l
   uint64_t s1 = ((uint64_t)funnel_shr(br.w2, br.w1, s1o) << 32) | funnel_shr(br.w1, br.w0, s1o);
   uint64_t s2 = (((uint64_t)in[3] << 32) | in[2]) >> (f.s2sh & 63u);
   // Texel 0's missing bit in both sets, then the anchors of set 1.
-  s1 = insert_zero64(s1, ~(uint64_t)(f.ins & 0xFFu));
+  s1 = insert_zero64(s1, 0xFFFFFFFF00000000ull | f.ins1);
   s1 = insert_zero64(s1, ((uint64_t)pt.ins[1] << 32) | pt.ins[0]);
   s1 = insert_zero64(s1, ((uint64_t)pt.ins[3] << 32) | pt.ins[2]);
-  s2 = insert_zero64(s2, ~(uint64_t)(f.ins >> 16));
+  s2 = insert_zero64(s2, 0xFFFFFFFF00000000ull | f.ins2);
   // Endpoints: field e of a channel at bit e * width of its chunk.
   const uint32_t cbx = f.wid, abx = f.wid >> 8, ibx = f.wid >> 16, ib2x = f.wid >> 24;
-  const uint32_t pks = f.ins >> 8;
+  // P-bit number of endpoint e: e, or e >> 1 for the shared p-bits of mode 1.
+  const uint32_t pks = bfe_u32(2u, mode, 1);
   // Field offsets e * width as running sums (in_vgpr keeps them additions:
   // the compiler would otherwise fold them into slower multiplies and shifts).
   uint32_t oc = 0u, oa = 0u;
@@ -388,13 +389,16 @@ BCN_HD void decode_bc7_rows(const uint32_t in[4], Bc7SubsetData* sub, uint32_t l
 #pragma unroll
     for (int h = 0; h < 2; h++) {
       const uint32_t e = 2u * (uint32_t)sb + (uint32_t)h;
-      if (e > 0) {
+      if (e == 1) {
+        oc = cbx;
+        oa = abx;
+      } else if (e > 1) {
         oc = in_vgpr(oc + cbx);
         oa = in_vgpr(oa + abx);
       }
       const uint32_t vrg = bfe_u32(chr, oc, cbx) | (bfe_u32(chg, oc, cbx) << 16);
       const uint32_t vba = bfe_u32(chb, oc, cbx) | (bfe_u32(cha, oa, abx) << 16);
-      const uint32_t pm = bit_mask(pbits, e >> (pks & 31u));
+      const uint32_t pm = bit_mask(pbits, e >> pks);
       rg[h] = pk_shr16(pk_mad16(vrg, f.mul_rg, pm & f.k_rg), f.bt_rg);
       ba[h] = pk_shr16(pk_mad16(vba, f.mul_ba, (pm & f.k_ba) | f.c_ba), f.bt_ba);
     }
@@ -414,8 +418,10 @@ BCN_HD void decode_bc7_rows(const uint32_t in[4], Bc7SubsetData* sub, uint32_t l
   uint32_t o1[8], o2[8];
   o1[0] = 0u;
   o2[0] = 0u;
+  o1[1] = ibx;
+  o2[1] = ib2x;
 #pragma unroll
-  for (int j = 1; j < 8; j++) {
+  for (int j = 2; j < 8; j++) {
     o1[j] = in_vgpr(o1[j - 1] + ibx);
     o2[j] = in_vgpr(o2[j - 1] + ib2x);
   }
