@@ -29,13 +29,18 @@ enum Bc6hOut : int {
   kOutF32RefRgb = 3   // the same in an RGBA32F image
 };
 
+// Workgroup size of decode_rgba8_kernel: BC7 runs one wave per group, so each
+// wave's LDS records are freed as soon as that wave ends.
+constexpr int rgba8_group(int fmt) { return fmt == BCN_BC7 ? 64 : kDecGroup; }
+
 template <int FMT>
-__global__ __launch_bounds__(kDecGroup) void decode_rgba8_kernel(const uint8_t* __restrict__ blocks,
-                                                                 BlockRows rows, uint32_t nblocks,
-                                                                 uint8_t* __restrict__ rgba,
-                                                                 uint32_t pitch) {
+__global__ __launch_bounds__(rgba8_group(FMT)) void decode_rgba8_kernel(const uint8_t* __restrict__ blocks,
+                                                                        BlockRows rows, uint32_t nblocks,
+                                                                        uint8_t* __restrict__ rgba,
+                                                                        uint32_t pitch) {
   constexpr int BB = (FMT == BCN_BC1 || FMT == BCN_BC4) ? 8 : 16;
-  const uint32_t idx = blockIdx.x * kDecGroup + threadIdx.x;
+  constexpr int G = rgba8_group(FMT);
+  const uint32_t idx = blockIdx.x * G + threadIdx.x;
   if (idx >= nblocks) return;
   const uint32_t* src = reinterpret_cast<const uint32_t*>(blocks + (size_t)idx * BB);
   uint32_t in[4];
@@ -44,9 +49,9 @@ __global__ __launch_bounds__(kDecGroup) void decode_rgba8_kernel(const uint8_t* 
   uint32_t yy, xx;
   block_pos(rows, idx, yy, xx);
   uint8_t* o = rgba + (size_t)yy * 4 * pitch + (size_t)xx * 16;
-  // BC7's three subset records per lane, interleaved by lane (12 KiB).
-  static_assert(kDecGroup == (int)kBc7SubRows, "the BC7 record addresses assume 256 lanes");
-  __shared__ Bc7SubsetData subs[FMT == BCN_BC7 ? 3 * kDecGroup : 1];
+  // BC7's three subset records per lane, interleaved by lane (3 KiB).
+  static_assert(rgba8_group(BCN_BC7) == (int)kBc7SubRows, "the BC7 record addresses assume 64 lanes");
+  __shared__ Bc7SubsetData subs[FMT == BCN_BC7 ? 3 * kBc7SubRows : 1];
   if constexpr (FMT == BCN_BC7) {
     // Each row is stored as soon as it is decoded.
     decode_bc7_rows(in, subs, threadIdx.x, [&](int y, uint32_t a, uint32_t b, uint32_t c, uint32_t e) {
@@ -428,7 +433,12 @@ int bcn_decode_rgba8(bcn_variant variant, const void* blocks, size_t blocks_size
     case BCN_BC5: hipLaunchKernelGGL(decode_rgba8_kernel<BCN_BC5>, grid, block, 0, s, b, rows, nblocks, o, rgba_pitch); break;
     case BCN_BC6H: hipLaunchKernelGGL(decode_bc6h_srgb8_kernel<false>, grid, block, 0, s, b, rows, nblocks, o, rgba_pitch); break;
     case BCN_BC6H_SF16: hipLaunchKernelGGL(decode_bc6h_srgb8_kernel<true>, grid, block, 0, s, b, rows, nblocks, o, rgba_pitch); break;
-    default: hipLaunchKernelGGL(decode_rgba8_kernel<BCN_BC7>, grid, block, 0, s, b, rows, nblocks, o, rgba_pitch); break;
+    default: {
+      constexpr int G = rgba8_group(BCN_BC7);
+      hipLaunchKernelGGL(decode_rgba8_kernel<BCN_BC7>, dim3((nblocks + G - 1) / G), dim3(G), 0, s, b, rows,
+                         nblocks, o, rgba_pitch);
+      break;
+    }
   }
   return launch_ok() ? BCN_OK : BCN_ERR_HIP;
 }

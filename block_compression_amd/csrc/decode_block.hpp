@@ -162,10 +162,11 @@ struct alignas(16) Bc7SubsetData {
   uint32_t crg, cba, drg, dba;
 };
 
-// The record of subset s for lane l (l < 256) is element s * kBc7SubRows + l
-// of the record array: byte address s * 4096 + 16 l, whose low byte comes
-// from the lane alone and whose second byte is s << 4 | l >> 4.
-constexpr uint32_t kBc7SubRows = 256;
+// The record of subset s for lane l (l < 64: one wave per workgroup) is
+// element s * kBc7SubRows + l of the record array: byte address
+// s * 1024 + 16 l, whose low byte comes from the lane alone and whose second
+// byte is s << 2 | l >> 4.
+constexpr uint32_t kBc7SubRows = 64;
 
 // Per-block constants of the BC7 decode, one 64-byte row per form:
 //   rows 0-3: modes 0-3; row 8: a first byte of 0 (every constant 0, so the
@@ -202,8 +203,8 @@ struct alignas(16) Bc7DecForm {
 };
 
 // Per partition kind (one subset; two or three subsets at 2 or 3 index bits)
-// and partition: the subset of texel 4 q + t in byte t of sub4[q] (times 16,
-// the record address's second byte), and the masks ~((1 << P) - 1) (64 bits,
+// and partition: the subset of texel 4 q + t in byte t of sub4[q] (times 4,
+// its part of the record address's second byte), and the masks ~((1 << P) - 1) (64 bits,
 // low word first) that insert the anchors' missing index bits at their
 // positions P = (a + 1) ib - 1 of the set 1 stream, in ascending order (0
 // inserts nothing; insert_zero64).
@@ -282,7 +283,7 @@ constexpr Bc7DecTables make_bc7_dec_tables() {
     for (int p = 0; p < 64; p++) {
       Bc7DecPart& r = d.part[16 + 64 * kind + p];
       const int pi = ns == 3 ? 64 + p : p;
-      for (int k = 0; k < 16; k++) r.sub4[k >> 2] |= ((kShapes[pi] >> (2 * k)) & 3u) << (8 * (k & 3) + 4);
+      for (int k = 0; k < 16; k++) r.sub4[k >> 2] |= ((kShapes[pi] >> (2 * k)) & 3u) << (8 * (k & 3) + 2);
       int a = (pt.anchor1[pi] + 1) * ib - 1, b = ns == 3 ? (pt.anchor2[pi] + 1) * ib - 1 : 64;
       if (b < a) { const int s = a; a = b; b = s; }
       bc7_ins_mask(a, r.ins[0], r.ins[1]);
