@@ -641,8 +641,8 @@ BCN_HD bool bc6h_one_subset(int m) { return m >= 10; }
 // what the texels need to read them (decode_bc6h_texel reads them in order),
 // and, for one-subset blocks (ONE), the subset's interpolation records.
 struct Bc6hIdx {
-  uint32_t j0, j1;   // index bits not yet read, lowest first
-  uint32_t amask;    // anchor texels (read one bit less)
+  uint32_t lo, mid;  // the index stream with the anchors' missing bits inserted: bits 0.., 8 ib..
+  uint32_t off[8];   // j * ib: index k at bit off[k & 7] of lo (k < 8) or mid
   uint32_t pattern;  // 2-bit subset ids of the 16 texels
   uint32_t ib;       // index bits
   int kw;            // weight multiplier: w = (i * kw + 32) >> 6
@@ -757,13 +757,22 @@ BCN_HD Bc6hIdx decode_bc6h_header(const uint32_t in[4], int m, Bc6hScratch sc, c
   st.w3 = in[3];
   st.pattern = two ? kPart.subset[part] : 0u;
   const uint32_t a1 = two ? kPart.anchor1[part] : 0u;
-  st.amask = 1u | (1u << a1);
   st.ib = two ? 3u : 4u;
   st.kw = two ? 585 : 273;
-  // The indices: the block's last 16 ib - (1 + two) bits (46 or 63).
+  // The indices: the block's last 16 ib - (1 + two) bits (46 or 63), made
+  // uniform by inserting a 0 at each anchor's missing top bit -- texel 0's at
+  // bit ib - 1, then subset 1's anchor a1 at (a1 + 1) ib - 1 -- so index k
+  // sits at bit k ib (insert_zero64).
   const uint32_t sh = two ? 18u : 1u;
-  st.j0 = funnel_shr(in[3], in[2], sh);
-  st.j1 = in[3] >> sh;
+  uint64_t s = (((uint64_t)in[3] << 32) | in[2]) >> sh;
+  s = insert_zero64(s, ~0ull << (st.ib - 1u));
+  s = insert_zero64(s, two ? ~0ull << ((a1 + 1u) * 3u - 1u) : 0ull);
+  st.lo = (uint32_t)s;
+  st.mid = (uint32_t)(s >> (8u * st.ib));
+  st.off[0] = 0u;
+  st.off[1] = st.ib;
+#pragma unroll
+  for (int j = 2; j < 8; j++) st.off[j] = in_vgpr(st.off[j - 1] + st.ib);
   return st;
 }
 
@@ -777,10 +786,7 @@ BCN_HD void decode_bc6h_texel(Bc6hIdx& st, int k, Bc6hScratch sc, uint16_t rgb[3
   if (ONE) {
     x = k == 0 ? bfe_u32(st.w2, 1, 3) : bfe_u32(k < 8 ? st.w2 : st.w3, 4u * (uint32_t)(k & 7), 4);
   } else {
-    const uint32_t n = st.ib - bfe_u32(st.amask, (uint32_t)k, 1);
-    x = bfe_u32(st.j0, 0, n);
-    st.j0 = funnel_shr(st.j1, st.j0, n);
-    st.j1 >>= n;
+    x = bfe_u32(k < 8 ? st.lo : st.mid, st.off[k & 7], st.ib);
     s = (int)bfe_u32(st.pattern, 2u * (uint32_t)k, 1);
   }
   const int w = (mul24((int)x, ONE ? 273 : st.kw) + 32) >> 6;
