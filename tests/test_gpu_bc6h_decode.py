@@ -64,6 +64,27 @@ def test_gpu_decode_matches_oracle_random_blocks(tag, name):
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("mode", list(range(9)))
+def test_gpu_decode_bc7_one_mode_per_image(mode):
+    """Random BC7 blocks all of one mode (8: a first byte of 0), so every wave
+    is uniform: the decoder's paths that skip subsets 1 and 2 for waves
+    without 2- or 3-subset blocks run, against the oracle's decoder."""
+    from block_compression_amd import decode_rgba8
+
+    w, h = 256, 64  # 1,024 blocks: 16 waves
+    nb = (w // 4) * (h // 4)
+    blocks = np.random.default_rng(70 + mode).integers(0, 256, (nb, 16), dtype=np.uint8)
+    if mode < 8:
+        blocks[:, 0] = (blocks[:, 0] & np.uint8((0xFF << (mode + 1)) & 0xFF)) | np.uint8(1 << mode)
+    else:
+        blocks[:, 0] = 0
+    blocks = blocks.reshape(-1)
+    v = CompressionVariant(6, BC7Settings.opaque_basic())
+    got = decode_rgba8(v, torch.from_numpy(blocks).cuda(), w, h).cpu().numpy()
+    ref = oracle_lib.decompress_rgba8(6, w, h, blocks)
+    assert np.array_equal(got, ref)
+
+
 @pytest.mark.parametrize("is_signed", [False, True])
 def test_gpu_decode_bc6h_random_blocks(is_signed):
     from block_compression_amd import decode_bc6h_rgba16f
@@ -79,6 +100,27 @@ def test_gpu_decode_bc6h_random_blocks(is_signed):
         tile = got_bits[yy * 4:yy * 4 + 4, xx * 4:xx * 4 + 4]
         assert np.array_equal(tile[..., :3], ref)
         assert (tile[..., 3] == 0x3C00).all()
+
+
+@pytest.mark.parametrize("is_signed", [False, True])
+def test_gpu_decode_bc6h_one_subset_waves(is_signed):
+    """Random BC6H blocks all of the one-subset modes (5-bit prefixes 0x03,
+    0x07, 0x0B, 0x0F), so every wave takes the decoders' one-subset path:
+    RGBA16F against the oracle's decoder, bit for bit."""
+    from block_compression_amd import decode_bc6h_rgba16f
+
+    w, h = 128, 64  # 512 blocks: 8 waves
+    nb = (w // 4) * (h // 4)
+    rng = np.random.default_rng(80 + is_signed)
+    blocks = rng.integers(0, 256, (nb, 16), dtype=np.uint8)
+    prefix = rng.choice(np.array([0x03, 0x07, 0x0B, 0x0F], dtype=np.uint8), nb)
+    blocks[:, 0] = (blocks[:, 0] & np.uint8(0xE0)) | prefix
+    got = decode_bc6h_rgba16f(torch.from_numpy(blocks.reshape(-1)).cuda(), w, h, is_signed).cpu().numpy()
+    got_bits = got.view(np.uint16)
+    for i, b in enumerate(blocks):
+        yy, xx = divmod(i, w // 4)
+        ref = oracle_lib.decode_block_bc6h(b.tobytes(), 12, is_signed)[:48].reshape(4, 4, 3)
+        assert np.array_equal(got_bits[yy * 4:yy * 4 + 4, xx * 4:xx * 4 + 4, :3], ref)
 
 
 def test_gpu_encode_decode_roundtrip_bc7_4096():
