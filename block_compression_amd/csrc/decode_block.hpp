@@ -345,7 +345,9 @@ BCN_HD void decode_bc7_rows(const uint32_t in[4], Bc7SubsetData* sub, uint32_t l
   const uint32_t pr = bfe_u32(in[0], mode + 1u, bfe_u32(kBc7Pbw, m4, 4));
   const uint32_t row = mode + bfe_u32(kBc7Adj, m4, 4) + (pr & bfe_u32(kBc7Vmask, m4, 4));
   // Both rows through one 32-bit byte offset from the table's base, each
-  // from the mode bits alone, so the two loads go out together.
+  // indexed by the mode and the bits after it alone (the partition row's
+  // base comes from the mode, not from the form row), so the two loads go
+  // out together.
   const uint32_t prow = bfe_u32(kBc7Prow16, m4, 4) << 4;
   const char* tab = reinterpret_cast<const char*>(&kBc7Dec);
   const Bc7DecForm f = *reinterpret_cast<const Bc7DecForm*>(tab + row * (uint32_t)sizeof(Bc7DecForm));

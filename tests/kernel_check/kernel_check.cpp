@@ -151,7 +151,7 @@ extern "C" int kcheck_decode(int fmt, int is_signed, const uint8_t* blocks, size
       continue;
     }
     uint32_t px[16];
-    // The BC7 decoder's record array, as one lane (0) of a 256-lane group.
+    // The BC7 decoder's record array, as lane 0 of a workgroup.
     static bcn::Bc7SubsetData sub[3 * bcn::kBc7SubRows];
     switch (fmt) {
       case 0: bcn::decode_rgba8_block<0>(in, px, sub, 0); break;
