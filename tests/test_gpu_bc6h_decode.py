@@ -64,6 +64,22 @@ def test_gpu_decode_matches_oracle_random_blocks(tag, name):
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("tag,name", [(0, "bc1"), (3, "bc4"), (6, "bc7")])
+def test_gpu_decode_partial_waves(tag, name):
+    """100x12 (25 blocks a row, 75 blocks): the last wave is partly empty and
+    block rows end inside waves, against the oracle's decoder."""
+    from block_compression_amd import decode_rgba8
+
+    w, h = 100, 12
+    bb = 8 if tag in (0, 3) else 16
+    nb = (w // 4) * (h // 4)
+    blocks = np.random.default_rng(90 + tag).integers(0, 256, nb * bb, dtype=np.uint8)
+    v = CompressionVariant(tag, BC7Settings.opaque_basic() if tag == 6 else None)
+    got = decode_rgba8(v, torch.from_numpy(blocks).cuda(), w, h).cpu().numpy()
+    ref = oracle_lib.decompress_rgba8(tag, w, h, blocks)
+    assert np.array_equal(got, ref)
+
+
 @pytest.mark.parametrize("mode", list(range(9)))
 def test_gpu_decode_bc7_one_mode_per_image(mode):
     """Random BC7 blocks all of one mode (8: a first byte of 0), so every wave
