@@ -67,16 +67,11 @@ __global__ __launch_bounds__(rgba8_group(FMT)) void decode_rgba8_kernel(const ui
         make_uint4(px[y * 4 + 0], px[y * 4 + 1], px[y * 4 + 2], px[y * 4 + 3]);
 }
 
-// The BC6H decoders' LDS: the decoder's field table (one copy per
-// workgroup) and each lane's 18-word scratch row, interleaved by lane.
+// The BC6H decoders' LDS: each lane's 18-word scratch row, interleaved by
+// lane.  (The field table is read from constant memory, a row per lane.)
 struct Bc6hLds {
-  uint16_t fields[kBc6hDecRows * 23];
   uint32_t scratch[18 * kDecGroup];
 };
-
-__device__ __forceinline__ void load_bc6h_fields(Bc6hLds& l) {
-  for (uint32_t i = threadIdx.x; i < kBc6hDecRows * 23; i += kDecGroup) l.fields[i] = (&kBc6hDec.f[0][0])[i];
-}
 
 // One lane's BC6H block to RGBA8 sRGB codes, row by row: each row is stored
 // as soon as it is decoded.  ONE: the wave holds only one-subset blocks.
@@ -117,7 +112,6 @@ __global__ __launch_bounds__(kDecGroup) void decode_bc6h_srgb8_kernel(
   __shared__ Bc6hLds lds;
   for (uint32_t i = threadIdx.x; i < kSrgb8Buckets / 8; i += kDecGroup)
     reinterpret_cast<uint4*>(lb)[i] = reinterpret_cast<const uint4*>(g_srgb8_buckets)[i];
-  load_bc6h_fields(lds);
   __syncthreads();
   const uint32_t idx = blockIdx.x * kDecGroup + threadIdx.x;
   if (idx >= nblocks) return;
@@ -129,9 +123,9 @@ __global__ __launch_bounds__(kDecGroup) void decode_bc6h_srgb8_kernel(
   uint8_t* o = rgba + (size_t)yy * 4 * pitch + (size_t)xx * 16;
   const int m = bc6h_mode(in);
   if (wave_all(bc6h_one_subset(m)))
-    bc6h_srgb8_rows<SGN, true>(in, m, sc, lds.fields, lb, o, pitch);
+    bc6h_srgb8_rows<SGN, true>(in, m, sc, &kBc6hDec.f[0][0], lb, o, pitch);
   else
-    bc6h_srgb8_rows<SGN, false>(in, m, sc, lds.fields, lb, o, pitch);
+    bc6h_srgb8_rows<SGN, false>(in, m, sc, &kBc6hDec.f[0][0], lb, o, pitch);
 }
 
 // One block row per lane (N x 16 bytes, lane order) stored through the
@@ -231,8 +225,6 @@ __global__ __launch_bounds__(kDecGroup) void decode_bc6h_kernel(const uint8_t* _
   constexpr bool kRun = OUT == kOutF16Rgba || OUT == kOutF32Rgba;
   constexpr int N = F32 ? 4 : 2;  // 16-byte chunks per block row
   __shared__ uint4 runs[kRun ? (kDecGroup / 64) * 64 * N : 1];
-  load_bc6h_fields(lds);
-  __syncthreads();
   const uint32_t idx = blockIdx.x * kDecGroup + threadIdx.x;
   if (idx >= nblocks) return;
   const uint32_t* src = reinterpret_cast<const uint32_t*>(blocks + (size_t)idx * 16);
@@ -251,9 +243,9 @@ __global__ __launch_bounds__(kDecGroup) void decode_bc6h_kernel(const uint8_t* _
   uint8_t* run_dst = rgba + (size_t)fy * 4 * pitch + (size_t)fx * (F32 ? 64 : 32);
   const int m = bc6h_mode(in);
   if (wave_all(bc6h_one_subset(m)))
-    bc6h_float_rows<OUT, SGN, true, N>(in, m, sc, lds.fields, o, pitch, run, slice, run_dst, lane);
+    bc6h_float_rows<OUT, SGN, true, N>(in, m, sc, &kBc6hDec.f[0][0], o, pitch, run, slice, run_dst, lane);
   else
-    bc6h_float_rows<OUT, SGN, false, N>(in, m, sc, lds.fields, o, pitch, run, slice, run_dst, lane);
+    bc6h_float_rows<OUT, SGN, false, N>(in, m, sc, &kBc6hDec.f[0][0], o, pitch, run, slice, run_dst, lane);
 }
 
 // ---- host side --------------------------------------------------------------

@@ -557,8 +557,10 @@ BCN_HD uint32_t reverse_low_bits(uint32_t v, uint32_t n) {
 // a read of no bits ORed into bit 0.  Row 14 is all 0, the row of the
 // reserved mode prefixes: no fields, so every component is 0.
 constexpr int kBc6hDecRows = 15;
-struct Bc6hDecFields {
-  uint16_t f[kBc6hDecRows][23];
+// Rows of 24 entries (48 bytes, the last entry 0), so a lane loads its
+// mode's row with three 16-byte reads.
+struct alignas(16) Bc6hDecFields {
+  uint16_t f[kBc6hDecRows][24];
 };
 constexpr Bc6hDecFields make_bc6h_dec_fields() {
   Bc6hDecFields t{};
@@ -660,7 +662,8 @@ struct Bc6hIdx {
 // lane's mode, and a field the mode lacks is a read of width 0.
 //   * Each endpoint field is read and ORed into its 16-bit slot in the lane's
 //     scratch row (an LDS OR on the GPU), rather than merged through selects
-//     over all 12 slots.  `fields` is kBc6hDec (a copy in LDS on the GPU).
+//     over all 12 slots.  `fields` is kBc6hDec, 16-byte aligned; the
+//     lane's row is read once into registers.
 //     The field loop ends once no lane of the wave has a field left.
 //   * A reserved mode prefix takes the field row of no fields: its
 //     components are all 0 and so is every texel (block.rs:282-290).
@@ -676,11 +679,17 @@ BCN_HD Bc6hIdx decode_bc6h_header(const uint32_t in[4], int m, Bc6hScratch sc, c
   br.skip((in[0] & 3u) > 1 ? 5u : 2u);
 #pragma unroll
   for (int w = 0; w < 6; w++) sc.at(w) = 0;
-  const uint16_t* ft = fields + m * 23;
+  // The mode's row, loaded whole into registers (two entries a word).
+  uint32_t rw[12];
+  {
+    const uint32_t* rp = static_cast<const uint32_t*>(__builtin_assume_aligned(fields + m * 24, 16));
+#pragma unroll
+    for (int q = 0; q < 12; q++) rw[q] = rp[q];
+  }
   // Every field is merged as read; the reversed ones are put in order below.
 #pragma unroll
   for (int i = 0; i < (ONE ? 9 : 23); i++) {
-    const uint32_t t = ft[i];
+    const uint32_t t = (rw[i >> 1] >> (16 * (i & 1))) & 0xFFFFu;
     if (!ONE && !wave_any(t != 0u)) break;  // entries past a mode's count are 0
     const uint32_t n = t & 31u, dst = (t >> 5) & 0xFFu;
     const uint32_t v = br.read(n);
